@@ -1,0 +1,629 @@
+/*
+ * dtfft_oracle.c -- CPU oracle for the single-GPU pencil-cycle hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP kernels of
+ * dtfft_amd are compared against.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it; the product path never does.
+ *
+ * Part 1 restates, loop for loop, the host transpose / pack / unpack routines
+ * of dtFFT (src/include/_dtfft_kernel_host_routines.inc).  Every routine moves
+ * whole elements of 4, 8 or 16 bytes, so results are bit-exact by
+ * construction.  Indices are 0-based here; the reference adds 1 for Fortran.
+ * `locals` is the 5-vector (nxx, nyy, nzz, din, dout) of one neighbour, the
+ * column layout of neighbor_data in src/dtfft_abstract_kernel.F90:86.
+ *
+ * Part 2 is an FP64 (and FP32, for the CPU baseline) mixed-radix Stockham FFT
+ * with the FFTW sign convention (forward exp(-2*pi*i*jk/n), backward
+ * exp(+...), unnormalised), the semantics dtFFT gets from
+ * src/interfaces/fft/fftw/dtfft_executor_fftw_m.F90:149-223.  Twiddles are
+ * computed in long double and rounded once.
+ *
+ * Parity pinning (see tests/test_oracle_*.py): the index maps are checked
+ * against numpy axis permutations and the reference's own identities
+ * (src/tests/test_host_kernels.F90); the FFT against numpy/pocketfft golden
+ * vectors, an O(n^2) long-double DFT and closed forms.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef struct { uint32_t a; } e4_t;
+typedef struct { uint64_t a; } e8_t;
+typedef struct { uint64_t a, b; } e16_t;
+
+/* Dispatch a loop body on element size.  BODY uses `T` as the element type. */
+#define ES_DISPATCH(es, BODY)                          \
+    do {                                               \
+        if ((es) == 4) { typedef e4_t T; BODY; }       \
+        else if ((es) == 8) { typedef e8_t T; BODY; }  \
+        else { typedef e16_t T; BODY; }                \
+    } while (0)
+
+/* ---------------------------------------------------------------------------
+ * Part 1: index maps.  dims[0] is the fastest dimension (Fortran order).
+ * ------------------------------------------------------------------------- */
+
+/* permute_forward_write, _dtfft_kernel_host_routines.inc:140-196.
+ * 2D: out(y + x*ny) = in(x + y*nx).  3D: out(y + z*ny + x*ny*nz) = in(x + y*nx + z*nx*ny). */
+void orc_permute_forward_write(const void* vin, void* vout, int es, int ndims, const int* dims)
+{
+    long nx = dims[0], ny = dims[1];
+    if (ndims == 2) {
+        ES_DISPATCH(es, {
+            const T* in = (const T*)vin; T* out = (T*)vout;
+            _Pragma("omp parallel for collapse(2) schedule(static)")
+            for (long x = 0; x < nx; x++)
+                for (long y = 0; y < ny; y++)
+                    out[x * ny + y] = in[y * nx + x];
+        });
+        return;
+    }
+    long nz = dims[2], nx_ny = nx * ny, ny_nz = ny * nz;
+    ES_DISPATCH(es, {
+        const T* in = (const T*)vin; T* out = (T*)vout;
+        _Pragma("omp parallel for collapse(3) schedule(static)")
+        for (long x = 0; x < nx; x++)
+            for (long z = 0; z < nz; z++)
+                for (long y = 0; y < ny; y++)
+                    out[x * ny_nz + z * ny + y] = in[z * nx_ny + x + y * nx];
+    });
+}
+
+/* permute_forward_read, _dtfft_kernel_host_routines.inc:201-257 (same map, read-contiguous loop order). */
+void orc_permute_forward_read(const void* vin, void* vout, int es, int ndims, const int* dims)
+{
+    long nx = dims[0], ny = dims[1];
+    if (ndims == 2) {
+        ES_DISPATCH(es, {
+            const T* in = (const T*)vin; T* out = (T*)vout;
+            _Pragma("omp parallel for collapse(2) schedule(static)")
+            for (long y = 0; y < ny; y++)
+                for (long x = 0; x < nx; x++)
+                    out[x * ny + y] = in[y * nx + x];
+        });
+        return;
+    }
+    long nz = dims[2], nx_ny = nx * ny, ny_nz = ny * nz;
+    ES_DISPATCH(es, {
+        const T* in = (const T*)vin; T* out = (T*)vout;
+        _Pragma("omp parallel for collapse(3) schedule(static)")
+        for (long z = 0; z < nz; z++)
+            for (long y = 0; y < ny; y++)
+                for (long x = 0; x < nx; x++)
+                    out[z * ny + x * ny_nz + y] = in[z * nx_ny + x + y * nx];
+    });
+}
+
+/* permute_backward_write, _dtfft_kernel_host_routines.inc:262-302.
+ * out(z + x*nz + y*nz*nx) = in(x + y*nx + z*nx*ny). */
+void orc_permute_backward_write(const void* vin, void* vout, int es, const int* dims)
+{
+    long nx = dims[0], ny = dims[1], nz = dims[2], nx_ny = nx * ny, nz_nx = nz * nx;
+    ES_DISPATCH(es, {
+        const T* in = (const T*)vin; T* out = (T*)vout;
+        _Pragma("omp parallel for collapse(3) schedule(static)")
+        for (long y = 0; y < ny; y++)
+            for (long x = 0; x < nx; x++)
+                for (long z = 0; z < nz; z++)
+                    out[y * nz_nx + x * nz + z] = in[y * nx + x + z * nx_ny];
+    });
+}
+
+/* permute_backward_read, _dtfft_kernel_host_routines.inc:307-346. */
+void orc_permute_backward_read(const void* vin, void* vout, int es, const int* dims)
+{
+    long nx = dims[0], ny = dims[1], nz = dims[2], nx_ny = nx * ny, nz_nx = nz * nx;
+    ES_DISPATCH(es, {
+        const T* in = (const T*)vin; T* out = (T*)vout;
+        _Pragma("omp parallel for collapse(3) schedule(static)")
+        for (long z = 0; z < nz; z++)
+            for (long y = 0; y < ny; y++)
+                for (long x = 0; x < nx; x++)
+                    out[y * nz_nx + z + x * nz] = in[z * nx_ny + y * nx + x];
+    });
+}
+
+/* permute_backward_start_write, _dtfft_kernel_host_routines.inc:351-390.
+ * out(z + y*nz + x*nz*ny) = in(x + y*nx + z*nx*ny). */
+void orc_permute_backward_start_write(const void* vin, void* vout, int es, const int* dims)
+{
+    long nx = dims[0], ny = dims[1], nz = dims[2], nx_ny = nx * ny, nz_ny = nz * ny;
+    ES_DISPATCH(es, {
+        const T* in = (const T*)vin; T* out = (T*)vout;
+        _Pragma("omp parallel for collapse(3) schedule(static)")
+        for (long x = 0; x < nx; x++)
+            for (long y = 0; y < ny; y++)
+                for (long z = 0; z < nz; z++)
+                    out[x * nz_ny + y * nz + z] = in[y * nx + x + z * nx_ny];
+    });
+}
+
+/* permute_backward_start_read, _dtfft_kernel_host_routines.inc:395-434. */
+void orc_permute_backward_start_read(const void* vin, void* vout, int es, const int* dims)
+{
+    long nx = dims[0], ny = dims[1], nz = dims[2], nx_ny = nx * ny, nz_ny = nz * ny;
+    ES_DISPATCH(es, {
+        const T* in = (const T*)vin; T* out = (T*)vout;
+        _Pragma("omp parallel for collapse(3) schedule(static)")
+        for (long z = 0; z < nz; z++)
+            for (long y = 0; y < ny; y++)
+                for (long x = 0; x < nx; x++)
+                    out[y * nz + z + x * nz_ny] = in[z * nx_ny + y * nx + x];
+    });
+}
+
+/* permute_backward_end_pipelined_write, _dtfft_kernel_host_routines.inc:439-482.
+ * out(dout + x + y*nx + z*nx*ny) = in(din + x + z*nxx + y*nzz*nxx) over the (nxx,nyy,nzz) box. */
+void orc_permute_backward_end_pipelined_write(const void* vin, void* vout, int es, const int* dims, const int* locals)
+{
+    long nx = dims[0], ny = dims[1];
+    long nxx = locals[0], nyy = locals[1], nzz = locals[2], din = locals[3], dout = locals[4];
+    long nx_ny = nx * ny, nzz_nxx = nzz * nxx;
+    ES_DISPATCH(es, {
+        const T* in = (const T*)vin; T* out = (T*)vout;
+        _Pragma("omp parallel for collapse(3) schedule(static)")
+        for (long z = 0; z < nzz; z++)
+            for (long y = 0; y < nyy; y++)
+                for (long x = 0; x < nxx; x++)
+                    out[dout + z * nx_ny + y * nx + x] = in[din + z * nxx + y * nzz_nxx + x];
+    });
+}
+
+/* permute_backward_end_pipelined_read, _dtfft_kernel_host_routines.inc:507-550. */
+void orc_permute_backward_end_pipelined_read(const void* vin, void* vout, int es, const int* dims, const int* locals)
+{
+    long nx = dims[0], ny = dims[1];
+    long nxx = locals[0], nyy = locals[1], nzz = locals[2], din = locals[3], dout = locals[4];
+    long nx_ny = nx * ny, nzz_nxx = nzz * nxx;
+    ES_DISPATCH(es, {
+        const T* in = (const T*)vin; T* out = (T*)vout;
+        _Pragma("omp parallel for collapse(3) schedule(static)")
+        for (long y = 0; y < nyy; y++)
+            for (long z = 0; z < nzz; z++)
+                for (long x = 0; x < nxx; x++)
+                    out[dout + y * nx + z * nx_ny + x] = in[din + y * nzz_nxx + z * nxx + x];
+    });
+}
+
+/* unpack_pipelined, _dtfft_kernel_host_routines.inc:575-636.
+ * out(dout + x + y*nx + z*nx*ny) = in(din + x + y*nxx + z*nxx*nyy). */
+void orc_unpack_pipelined(const void* vin, void* vout, int es, int ndims, const int* dims, const int* locals)
+{
+    long nx = dims[0], ny = dims[1];
+    long nxx = locals[0], nyy = locals[1], din = locals[3], dout = locals[4];
+    long nzz = ndims == 2 ? 1 : locals[2];
+    long nx_ny = nx * ny, nxx_nyy = nxx * nyy;
+    ES_DISPATCH(es, {
+        const T* in = (const T*)vin; T* out = (T*)vout;
+        _Pragma("omp parallel for collapse(2) schedule(static)")
+        for (long z = 0; z < nzz; z++)
+            for (long y = 0; y < nyy; y++)
+                for (long x = 0; x < nxx; x++)
+                    out[dout + z * nx_ny + y * nx + x] = in[din + z * nxx_nyy + y * nxx + x];
+    });
+}
+
+/* pack_pipelined, _dtfft_kernel_host_routines.inc:657-721.
+ * out(dout + x + y*nxx + z*nxx*nyy) = in(din + x + y*nx + z*nx*ny). */
+void orc_pack_pipelined(const void* vin, void* vout, int es, int ndims, const int* dims, const int* locals)
+{
+    long nx = dims[0], ny = dims[1];
+    long nxx = locals[0], nyy = locals[1], din = locals[3], dout = locals[4];
+    long nzz = ndims == 2 ? 1 : locals[2];
+    long nx_ny = nx * ny, nxx_nyy = nxx * nyy;
+    ES_DISPATCH(es, {
+        const T* in = (const T*)vin; T* out = (T*)vout;
+        _Pragma("omp parallel for collapse(2) schedule(static)")
+        for (long z = 0; z < nzz; z++)
+            for (long y = 0; y < nyy; y++)
+                for (long x = 0; x < nxx; x++)
+                    out[dout + z * nxx_nyy + y * nxx + x] = in[din + z * nx_ny + y * nx + x];
+    });
+}
+
+/* copy, _dtfft_kernel_host_routines.inc:742-752: out(dout + i) = in(din + i), i < n. */
+void orc_copy(const void* vin, void* vout, int es, long n, long din, long dout)
+{
+    memcpy((char*)vout + dout * es, (const char*)vin + din * es, (size_t)n * es);
+}
+
+/* unpack_forward_pipelined_write, _dtfft_kernel_host_routines.inc:757-811.
+ * 2D: out(dout + y*nx + x) = in(din + x*nyy + y).
+ * 3D: out(dout + x + y*nx + z*nx*ny) = in(din + z + x*nzz + y*nzz*nxx). */
+void orc_unpack_forward_pipelined_write(const void* vin, void* vout, int es, int ndims, const int* dims, const int* locals)
+{
+    long nx = dims[0], ny = dims[1];
+    long nxx = locals[0], nyy = locals[1], din = locals[3], dout = locals[4];
+    if (ndims == 2) {
+        ES_DISPATCH(es, {
+            const T* in = (const T*)vin; T* out = (T*)vout;
+            _Pragma("omp parallel for collapse(2) schedule(static)")
+            for (long y = 0; y < nyy; y++)
+                for (long x = 0; x < nxx; x++)
+                    out[dout + y * nx + x] = in[din + x * nyy + y];
+        });
+        return;
+    }
+    long nzz = locals[2], nzz_nxx = nzz * nxx, ny_nx = ny * nx;
+    ES_DISPATCH(es, {
+        const T* in = (const T*)vin; T* out = (T*)vout;
+        _Pragma("omp parallel for collapse(3) schedule(static)")
+        for (long z = 0; z < nzz; z++)
+            for (long y = 0; y < nyy; y++)
+                for (long x = 0; x < nxx; x++)
+                    out[dout + z * ny_nx + y * nx + x] = in[din + z + y * nzz_nxx + x * nzz];
+    });
+}
+
+/* unpack_forward_pipelined_read, _dtfft_kernel_host_routines.inc:816-870. */
+void orc_unpack_forward_pipelined_read(const void* vin, void* vout, int es, int ndims, const int* dims, const int* locals)
+{
+    long nx = dims[0], ny = dims[1];
+    long nxx = locals[0], nyy = locals[1], din = locals[3], dout = locals[4];
+    if (ndims == 2) {
+        ES_DISPATCH(es, {
+            const T* in = (const T*)vin; T* out = (T*)vout;
+            _Pragma("omp parallel for collapse(2) schedule(static)")
+            for (long x = 0; x < nxx; x++)
+                for (long y = 0; y < nyy; y++)
+                    out[dout + y * nx + x] = in[din + x * nyy + y];
+        });
+        return;
+    }
+    long nzz = locals[2], nxx_nzz = nxx * nzz, nx_ny = nx * ny;
+    ES_DISPATCH(es, {
+        const T* in = (const T*)vin; T* out = (T*)vout;
+        _Pragma("omp parallel for collapse(3) schedule(static)")
+        for (long y = 0; y < nyy; y++)
+            for (long x = 0; x < nxx; x++)
+                for (long z = 0; z < nzz; z++)
+                    out[dout + y * nx + x + z * nx_ny] = in[din + y * nxx_nzz + x * nzz + z];
+    });
+}
+
+/* unpack_backward_pipelined_write, _dtfft_kernel_host_routines.inc:907-946.
+ * out(dout + x + y*nx + z*nx*ny) = in(din + y + z*nyy + x*nzz*nyy). */
+void orc_unpack_backward_pipelined_write(const void* vin, void* vout, int es, const int* dims, const int* locals)
+{
+    long nx = dims[0], ny = dims[1];
+    long nxx = locals[0], nyy = locals[1], nzz = locals[2], din = locals[3], dout = locals[4];
+    long nzz_nyy = nzz * nyy, ny_nx = ny * nx;
+    ES_DISPATCH(es, {
+        const T* in = (const T*)vin; T* out = (T*)vout;
+        _Pragma("omp parallel for collapse(3) schedule(static)")
+        for (long z = 0; z < nzz; z++)
+            for (long y = 0; y < nyy; y++)
+                for (long x = 0; x < nxx; x++)
+                    out[dout + z * ny_nx + y * nx + x] = in[din + z * nyy + y + x * nzz_nyy];
+    });
+}
+
+/* unpack_backward_pipelined_read, _dtfft_kernel_host_routines.inc:951-989. */
+void orc_unpack_backward_pipelined_read(const void* vin, void* vout, int es, const int* dims, const int* locals)
+{
+    long nx = dims[0], ny = dims[1];
+    long nxx = locals[0], nyy = locals[1], nzz = locals[2], din = locals[3], dout = locals[4];
+    long nzz_nyy = nzz * nyy, nx_ny = nx * ny;
+    ES_DISPATCH(es, {
+        const T* in = (const T*)vin; T* out = (T*)vout;
+        _Pragma("omp parallel for collapse(3) schedule(static)")
+        for (long x = 0; x < nxx; x++)
+            for (long z = 0; z < nzz; z++)
+                for (long y = 0; y < nyy; y++)
+                    out[dout + x + z * nx_ny + y * nx] = in[din + x * nzz_nyy + z * nyy + y];
+    });
+}
+
+/* pack_forward_write, _dtfft_kernel_host_routines.inc:1026-1086.
+ * 2D: out(dout + x*nyy + y) = in(din + y*nx + x).
+ * 3D: out(dout + y + z*nyy + x*nyy*nzz) = in(din + x + y*nx + z*nx*ny). */
+void orc_pack_forward_write(const void* vin, void* vout, int es, int ndims, const int* dims, const int* locals)
+{
+    long nx = dims[0], ny = dims[1];
+    long nxx = locals[0], nyy = locals[1], din = locals[3], dout = locals[4];
+    if (ndims == 2) {
+        ES_DISPATCH(es, {
+            const T* in = (const T*)vin; T* out = (T*)vout;
+            _Pragma("omp parallel for collapse(2) schedule(static)")
+            for (long x = 0; x < nxx; x++)
+                for (long y = 0; y < nyy; y++)
+                    out[dout + x * nyy + y] = in[din + y * nx + x];
+        });
+        return;
+    }
+    long nzz = locals[2], nx_ny = nx * ny, nyy_nzz = nyy * nzz;
+    ES_DISPATCH(es, {
+        const T* in = (const T*)vin; T* out = (T*)vout;
+        _Pragma("omp parallel for collapse(3) schedule(static)")
+        for (long x = 0; x < nxx; x++)
+            for (long z = 0; z < nzz; z++)
+                for (long y = 0; y < nyy; y++)
+                    out[dout + x * nyy_nzz + z * nyy + y] = in[din + z * nx_ny + x + y * nx];
+    });
+}
+
+/* pack_forward_read, _dtfft_kernel_host_routines.inc:1091-1150. */
+void orc_pack_forward_read(const void* vin, void* vout, int es, int ndims, const int* dims, const int* locals)
+{
+    long nx = dims[0], ny = dims[1];
+    long nxx = locals[0], nyy = locals[1], din = locals[3], dout = locals[4];
+    if (ndims == 2) {
+        ES_DISPATCH(es, {
+            const T* in = (const T*)vin; T* out = (T*)vout;
+            _Pragma("omp parallel for collapse(2) schedule(static)")
+            for (long y = 0; y < nyy; y++)
+                for (long x = 0; x < nxx; x++)
+                    out[dout + x * nyy + y] = in[din + y * nx + x];
+        });
+        return;
+    }
+    long nzz = locals[2], nx_ny = nx * ny, nyy_nzz = nyy * nzz;
+    ES_DISPATCH(es, {
+        const T* in = (const T*)vin; T* out = (T*)vout;
+        _Pragma("omp parallel for collapse(3) schedule(static)")
+        for (long z = 0; z < nzz; z++)
+            for (long y = 0; y < nyy; y++)
+                for (long x = 0; x < nxx; x++)
+                    out[dout + z * nyy + x * nyy_nzz + y] = in[din + z * nx_ny + y * nx + x];
+    });
+}
+
+/* pack_backward_write, _dtfft_kernel_host_routines.inc:1155-1198.
+ * out(dout + z + x*nzz + y*nzz*nxx) = in(din + x + y*nx + z*nx*ny). */
+void orc_pack_backward_write(const void* vin, void* vout, int es, const int* dims, const int* locals)
+{
+    long nx = dims[0], ny = dims[1];
+    long nxx = locals[0], nyy = locals[1], nzz = locals[2], din = locals[3], dout = locals[4];
+    long nx_ny = nx * ny, nzz_nxx = nzz * nxx;
+    ES_DISPATCH(es, {
+        const T* in = (const T*)vin; T* out = (T*)vout;
+        _Pragma("omp parallel for collapse(3) schedule(static)")
+        for (long y = 0; y < nyy; y++)
+            for (long x = 0; x < nxx; x++)
+                for (long z = 0; z < nzz; z++)
+                    out[dout + y * nzz_nxx + x * nzz + z] = in[din + y * nx + x + z * nx_ny];
+    });
+}
+
+/* pack_backward_read, _dtfft_kernel_host_routines.inc:1203-1245. */
+void orc_pack_backward_read(const void* vin, void* vout, int es, const int* dims, const int* locals)
+{
+    long nx = dims[0], ny = dims[1];
+    long nxx = locals[0], nyy = locals[1], nzz = locals[2], din = locals[3], dout = locals[4];
+    long nx_ny = nx * ny, nzz_nxx = nzz * nxx;
+    ES_DISPATCH(es, {
+        const T* in = (const T*)vin; T* out = (T*)vout;
+        _Pragma("omp parallel for collapse(3) schedule(static)")
+        for (long z = 0; z < nzz; z++)
+            for (long y = 0; y < nyy; y++)
+                for (long x = 0; x < nxx; x++)
+                    out[dout + y * nzz_nxx + z + x * nzz] = in[din + z * nx_ny + y * nx + x];
+    });
+}
+
+/* ---------------------------------------------------------------------------
+ * Part 2: mixed-radix Stockham FFT (FP64 oracle; FP32 twin for the CPU
+ * baseline).  Factors: 4, 2, 3, 5, 7, then any remaining prime as a generic
+ * O(p^2) radix.  Pass p reads x[j + r*n/R], multiplies by w_{L*R}^{r*(j%L)},
+ * does an R-point DFT and writes y[(j - j%L)*R + j%L + s*L].
+ * ------------------------------------------------------------------------- */
+
+#define ORC_MAXF 64
+
+typedef struct {
+    int n, nf;
+    int radix[ORC_MAXF];
+    long double* w;   /* w[m] = exp(-2*pi*i*m/n), m < n, as (re, im) pairs */
+} orc_fft_plan;
+
+static int orc_factorize(int n, int* radix)
+{
+    int nf = 0;
+    while (n % 4 == 0) { radix[nf++] = 4; n /= 4; }
+    while (n % 2 == 0) { radix[nf++] = 2; n /= 2; }
+    for (int p = 3; n > 1; p += 2)
+        while (n % p == 0) { radix[nf++] = p; n /= p; }
+    return nf;
+}
+
+/* exp(-2*pi*i*m/n) with the angle reduced to the first octant in long double. */
+static void orc_root(long m, long n, long double* re, long double* im)
+{
+    m %= n; if (m < 0) m += n;
+    /* use symmetry so that the argument of cosl/sinl stays small */
+    const long double pi = 3.141592653589793238462643383279502884L;
+    long double a = 2.0L * pi * (long double)m / (long double)n;
+    *re = cosl(a);
+    *im = -sinl(a);
+    /* exact values at the quarter points */
+    if ((4 * m) % n == 0) {
+        long q = (4 * m) / n;
+        *re = (q == 0) ? 1.0L : (q == 2 ? -1.0L : 0.0L);
+        *im = (q == 1) ? -1.0L : (q == 3 ? 1.0L : 0.0L);
+    }
+}
+
+static void orc_plan_init(orc_fft_plan* p, int n)
+{
+    p->n = n;
+    p->nf = orc_factorize(n, p->radix);
+    p->w = (long double*)malloc(sizeof(long double) * 2 * (size_t)n);
+    for (int m = 0; m < n; m++) orc_root(m, n, &p->w[2 * m], &p->w[2 * m + 1]);
+}
+
+static void orc_plan_free(orc_fft_plan* p) { free(p->w); }
+
+#define ORC_DEFINE_FFT(REAL, SUFFIX)                                                        \
+static void orc_fft_line_##SUFFIX(const orc_fft_plan* p, REAL* x, REAL* y, int sign)       \
+{                                                                                            \
+    /* x: input / work (interleaved complex, length n), y: scratch; result left in x */    \
+    const int n = p->n;                                                                     \
+    REAL* src = x; REAL* dst = y;                                                           \
+    long L = 1;                                                                             \
+    REAL dft_re[64], dft_im[64];                                                            \
+    for (int f = 0; f < p->nf; f++) {                                                       \
+        const int R = p->radix[f];                                                          \
+        const long nR = n / R, LR = L * R, wstep = n / LR;                                  \
+        if (R != 2 && R != 4) {                                                             \
+            for (int q = 0; q < R; q++) {                                                   \
+                long m = (long)q * (n / R);                                                 \
+                dft_re[q] = (REAL)p->w[2 * (m % n)];                                        \
+                dft_im[q] = (REAL)p->w[2 * (m % n) + 1] * (REAL)(sign < 0 ? 1 : -1);        \
+            }                                                                               \
+        }                                                                                   \
+        for (long j = 0; j < nR; j++) {                                                     \
+            const long k = j % L;                                                           \
+            REAL ar[64], ai[64];                                                            \
+            for (int r = 0; r < R; r++) {                                                   \
+                REAL vr = src[2 * (j + r * nR)], vi = src[2 * (j + r * nR) + 1];            \
+                if (r && k) {                                                               \
+                    long m = ((long)r * k * wstep) % n;                                     \
+                    REAL wr = (REAL)p->w[2 * m];                                            \
+                    REAL wi = (REAL)p->w[2 * m + 1] * (REAL)(sign < 0 ? 1 : -1);            \
+                    REAL tr = vr * wr - vi * wi, ti = vr * wi + vi * wr;                    \
+                    vr = tr; vi = ti;                                                       \
+                }                                                                           \
+                ar[r] = vr; ai[r] = vi;                                                     \
+            }                                                                               \
+            const long base = (j - k) * R + k;                                              \
+            if (R == 2) {                                                                   \
+                dst[2 * base] = ar[0] + ar[1]; dst[2 * base + 1] = ai[0] + ai[1];           \
+                dst[2 * (base + L)] = ar[0] - ar[1];                                        \
+                dst[2 * (base + L) + 1] = ai[0] - ai[1];                                    \
+            } else if (R == 4) {                                                            \
+                REAL s0r = ar[0] + ar[2], s0i = ai[0] + ai[2];                              \
+                REAL d0r = ar[0] - ar[2], d0i = ai[0] - ai[2];                              \
+                REAL s1r = ar[1] + ar[3], s1i = ai[1] + ai[3];                              \
+                REAL d1r = ar[1] - ar[3], d1i = ai[1] - ai[3];                              \
+                /* multiply d1 by -i*sign... forward (sign<0): -i ; backward: +i */         \
+                REAL rr, ri;                                                                \
+                if (sign < 0) { rr = d1i; ri = -d1r; } else { rr = -d1i; ri = d1r; }        \
+                dst[2 * base] = s0r + s1r; dst[2 * base + 1] = s0i + s1i;                   \
+                dst[2 * (base + L)] = d0r + rr; dst[2 * (base + L) + 1] = d0i + ri;         \
+                dst[2 * (base + 2 * L)] = s0r - s1r;                                        \
+                dst[2 * (base + 2 * L) + 1] = s0i - s1i;                                    \
+                dst[2 * (base + 3 * L)] = d0r - rr;                                         \
+                dst[2 * (base + 3 * L) + 1] = d0i - ri;                                     \
+            } else {                                                                        \
+                for (int s = 0; s < R; s++) {                                               \
+                    REAL br = 0, bi = 0;                                                    \
+                    for (int r = 0; r < R; r++) {                                           \
+                        int q = (r * s) % R;                                                \
+                        br += ar[r] * dft_re[q] - ai[r] * dft_im[q];                        \
+                        bi += ar[r] * dft_im[q] + ai[r] * dft_re[q];                        \
+                    }                                                                       \
+                    dst[2 * (base + s * L)] = br; dst[2 * (base + s * L) + 1] = bi;         \
+                }                                                                           \
+            }                                                                               \
+        }                                                                                   \
+        REAL* t = src; src = dst; dst = t;                                                  \
+        L = LR;                                                                             \
+    }                                                                                       \
+    if (src != x) memcpy(x, src, sizeof(REAL) * 2 * (size_t)n);                             \
+}                                                                                            \
+                                                                                             \
+/* Batched contiguous C2C: howmany lines of n complex values, idist = n. */                 \
+void orc_fft_c2c_##SUFFIX(const REAL* in, REAL* out, int n, long howmany, int sign)        \
+{                                                                                            \
+    orc_fft_plan p; orc_plan_init(&p, n);                                                   \
+    _Pragma("omp parallel")                                                                 \
+    {                                                                                        \
+        REAL* work = (REAL*)malloc(sizeof(REAL) * 2 * (size_t)n);                           \
+        _Pragma("omp for schedule(static)")                                                 \
+        for (long b = 0; b < howmany; b++) {                                                \
+            REAL* o = out + 2 * (size_t)n * b;                                              \
+            if (o != in + 2 * (size_t)n * b)                                                \
+                memcpy(o, in + 2 * (size_t)n * b, sizeof(REAL) * 2 * (size_t)n);            \
+            orc_fft_line_##SUFFIX(&p, o, work, sign);                                       \
+        }                                                                                    \
+        free(work);                                                                          \
+    }                                                                                        \
+    orc_plan_free(&p);                                                                      \
+}                                                                                            \
+                                                                                             \
+/* Batched R2C: real lines of n (idist = n) -> n/2+1 complex (odist = n/2+1).   */          \
+void orc_fft_r2c_##SUFFIX(const REAL* in, REAL* out, int n, long howmany)                  \
+{                                                                                            \
+    orc_fft_plan p; orc_plan_init(&p, n);                                                   \
+    const long nc = n / 2 + 1;                                                              \
+    _Pragma("omp parallel")                                                                 \
+    {                                                                                        \
+        REAL* work = (REAL*)malloc(sizeof(REAL) * 4 * (size_t)n);                           \
+        _Pragma("omp for schedule(static)")                                                 \
+        for (long b = 0; b < howmany; b++) {                                                \
+            const REAL* i = in + (size_t)n * b;                                             \
+            for (long t = 0; t < n; t++) { work[2 * t] = i[t]; work[2 * t + 1] = 0; }       \
+            orc_fft_line_##SUFFIX(&p, work, work + 2 * (size_t)n, -1);                      \
+            memcpy(out + 2 * (size_t)nc * b, work, sizeof(REAL) * 2 * (size_t)nc);          \
+        }                                                                                    \
+        free(work);                                                                          \
+    }                                                                                        \
+    orc_plan_free(&p);                                                                      \
+}                                                                                            \
+                                                                                             \
+/* Batched C2R (FFTW semantics: the Hermitian extension of the n/2+1 inputs is     */       \
+/* transformed with the backward sign and the real part kept, so the imaginary    */       \
+/* parts of X[0] and X[n/2] do not contribute).                                    */       \
+void orc_fft_c2r_##SUFFIX(const REAL* in, REAL* out, int n, long howmany)                  \
+{                                                                                            \
+    orc_fft_plan p; orc_plan_init(&p, n);                                                   \
+    const long nc = n / 2 + 1;                                                              \
+    _Pragma("omp parallel")                                                                 \
+    {                                                                                        \
+        REAL* work = (REAL*)malloc(sizeof(REAL) * 4 * (size_t)n);                           \
+        _Pragma("omp for schedule(static)")                                                 \
+        for (long b = 0; b < howmany; b++) {                                                \
+            const REAL* i = in + 2 * (size_t)nc * b;                                        \
+            for (long k = 0; k < nc; k++) { work[2 * k] = i[2 * k]; work[2 * k + 1] = i[2 * k + 1]; } \
+            for (long k = 1; k < n - k; k++) {                                              \
+                work[2 * (n - k)] = i[2 * k]; work[2 * (n - k) + 1] = -i[2 * k + 1];        \
+            }                                                                                \
+            orc_fft_line_##SUFFIX(&p, work, work + 2 * (size_t)n, +1);                      \
+            REAL* o = out + (size_t)n * b;                                                  \
+            for (long t = 0; t < n; t++) o[t] = work[2 * t];                                \
+        }                                                                                    \
+        free(work);                                                                          \
+    }                                                                                        \
+    orc_plan_free(&p);                                                                      \
+}
+
+ORC_DEFINE_FFT(double, f64)
+ORC_DEFINE_FFT(float, f32)
+
+/* O(n^2) DFT in long double: the pin for the Stockham oracle on small lengths. */
+void orc_dft_naive(const double* in, double* out, int n, int sign)
+{
+    for (long k = 0; k < n; k++) {
+        long double sr = 0, si = 0;
+        for (long j = 0; j < n; j++) {
+            long double wr, wi;
+            orc_root(j * k, n, &wr, &wi);
+            if (sign > 0) wi = -wi;
+            sr += (long double)in[2 * j] * wr - (long double)in[2 * j + 1] * wi;
+            si += (long double)in[2 * j] * wi + (long double)in[2 * j + 1] * wr;
+        }
+        out[2 * k] = (double)sr; out[2 * k + 1] = (double)si;
+    }
+}
+
+int orc_max_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
