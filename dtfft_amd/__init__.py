@@ -1,0 +1,357 @@
+"""dtfft_amd -- MI355X (gfx950) implementation of dtFFT's single-GPU hot path.
+
+Python mirror of the dtFFT plugin and plan interfaces over the C-ABI of
+``libdtfft_hip.so`` (``include/dtfft_hip.h``):
+
+* :class:`Kernel`   -- kernel plugin (permute / pack / unpack), dtFFT ``kernel_device``;
+* :class:`Executor` -- FFT executor plugin (batched 1D/2D C2C, R2C/C2R), dtFFT
+  ``abstract_executor``;
+* :class:`PlanC2C`, :class:`PlanR2C` -- single-rank plans with dtFFT's Python API
+  names (``execute``, ``transpose``, ``local_sizes``, ``alloc_size``, ...);
+* :class:`Cycle`    -- many independent grids in one persistent launch per stage;
+* :class:`DeviceArray` -- a device buffer with numpy upload / download.
+
+All compute runs in hand-written HIP kernels compiled by hipRTC for gfx950; there is
+no CPU fallback (the library raises if it is not built or no gfx950 GPU is present).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+
+import numpy as np
+
+from ._lib import Config, DtfftError, call, check, lib  # noqa: F401
+
+__all__ = ["Precision", "Effort", "Executor", "Execute", "Transpose", "KernelType", "FFTType",
+           "DeviceArray", "Kernel", "Executor", "FFTExecutor", "PlanC2C", "PlanR2C", "Cycle",
+           "Config", "DtfftError", "set_config", "device_arch", "synchronize"]
+
+
+class Precision(enum.IntEnum):
+    SINGLE = 0
+    DOUBLE = 1
+
+
+class Effort(enum.IntEnum):
+    ESTIMATE = 0
+    MEASURE = 1
+    PATIENT = 2
+    EXHAUSTIVE = 3
+
+
+class Executor(enum.IntEnum):
+    NONE = 0
+    HIP = 5
+
+
+class Execute(enum.IntEnum):
+    FORWARD = 11
+    BACKWARD = 12
+
+
+class Transpose(enum.IntEnum):
+    X_TO_Y = 1
+    Y_TO_X = -1
+    Y_TO_Z = 2
+    Z_TO_Y = -2
+    X_TO_Z = 3
+    Z_TO_X = -3
+
+
+class KernelType(enum.IntEnum):
+    """kernel_type_t, src/dtfft_abstract_kernel.F90:59-98"""
+    DUMMY = -1
+    PACK = 1
+    COPY_PIPELINED = 2
+    UNPACK = 3
+    COPY = 4
+    UNPACK_PIPELINED = 5
+    PACK_PIPELINED = 6
+    PERMUTE_FORWARD = 7
+    PERMUTE_BACKWARD = 8
+    PERMUTE_BACKWARD_START = 9
+    PERMUTE_BACKWARD_END = 10
+    PERMUTE_BACKWARD_END_PIPELINED = 11
+    PACK_FORWARD = 12
+    PACK_BACKWARD = 13
+    UNPACK_FORWARD = 15
+    UNPACK_FORWARD_PIPELINED = 16
+    UNPACK_BACKWARD = 17
+    UNPACK_BACKWARD_PIPELINED = 18
+
+
+class FFTType(enum.IntEnum):
+    C2C = 0
+    R2C = 1
+    R2R = 2
+
+
+FFT_FORWARD = -1
+FFT_BACKWARD = 1
+
+
+def _ptr(x) -> int | None:
+    """Device address of a DeviceArray, an int, or an object exposing data_ptr()."""
+    if x is None:
+        return None
+    if isinstance(x, DeviceArray):
+        return x.ptr
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        return int(x.data_ptr())
+    if hasattr(x, "__cuda_array_interface__"):
+        return int(x.__cuda_array_interface__["data"][0])
+    raise TypeError(f"cannot take a device address of {type(x)!r}")
+
+
+class DeviceArray:
+    """Owning device buffer (hipMalloc through the library) with numpy transfers."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        call("dtfft_hip_mem_alloc", C.c_size_t(max(1, self.nbytes)), C.byref(p))
+        self.ptr = int(p.value)
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray, nbytes: int | None = None) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(max(a.nbytes, nbytes or 0))
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray, offset: int = 0) -> None:
+        a = np.ascontiguousarray(a)
+        assert offset + a.nbytes <= self.nbytes
+        call("dtfft_hip_memcpy_h2d", C.c_void_p(self.ptr + offset), a.ctypes.data_as(C.c_void_p),
+             C.c_size_t(a.nbytes))
+
+    def download(self, shape, dtype, offset: int = 0) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        assert offset + out.nbytes <= self.nbytes
+        call("dtfft_hip_memcpy_d2h", out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr + offset),
+             C.c_size_t(out.nbytes))
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().dtfft_hip_mem_free(C.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize() -> None:
+    call("dtfft_hip_device_synchronize")
+
+
+def device_arch() -> str:
+    buf = C.create_string_buffer(128)
+    call("dtfft_hip_get_device_arch", buf, 128)
+    return buf.value.decode()
+
+
+def set_config(enable_z_slab: bool = False, enable_fused: bool = True, enable_kernel_autotune: bool = False,
+               measure_warmup_iters: int = 2, measure_iters: int = 5, stream: int | None = None) -> None:
+    """dtfft_set_config for this library (src/dtfft_config.F90)."""
+    c = Config(int(enable_z_slab), int(enable_fused), int(enable_kernel_autotune), measure_warmup_iters,
+               measure_iters, stream)
+    call("dtfft_hip_set_config", C.byref(c))
+
+
+def _i32arr(v):
+    return (C.c_int32 * len(v))(*[int(x) for x in v])
+
+
+class Kernel:
+    """Kernel plugin handle (dtfft_hip_kernel_*), the drop-in for kernel_device.
+
+    ``dims`` are the local dims, fastest first; ``neighbor_data`` is a sequence of
+    (nxx, nyy, nzz, din, dout) rows, one per peer."""
+
+    def __init__(self, kernel_type, dims, base_storage: int, neighbor_data=None,
+                 effort: Effort = Effort.ESTIMATE):
+        self.h = C.c_void_p()
+        nd = None
+        npeers = 0
+        if neighbor_data is not None:
+            rows = [list(map(int, r)) for r in neighbor_data]
+            npeers = len(rows)
+            nd = _i32arr([x for r in rows for x in r])
+        call("dtfft_hip_kernel_create", int(kernel_type), len(dims), _i32arr(dims), int(base_storage), nd,
+             npeers, int(effort), 0, C.byref(self.h))
+
+    def execute(self, inp, out, neighbor: int = 0, stream: int | None = None, sync: bool = False) -> None:
+        call("dtfft_hip_kernel_execute", self.h, C.c_void_p(_ptr(inp)), C.c_void_p(_ptr(out)),
+             C.c_void_p(stream), int(neighbor), int(sync))
+
+    def destroy(self) -> None:
+        if self.h:
+            lib().dtfft_hip_kernel_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class FFTExecutor:
+    """FFT executor plugin handle (dtfft_hip_fft_*), the drop-in for abstract_executor."""
+
+    def __init__(self, fft_rank: int, fft_type: FFTType, precision: Precision, idist: int, odist: int,
+                 how_many: int, fft_sizes, inembed=None, onembed=None, effort: Effort = Effort.ESTIMATE,
+                 stream: int | None = None):
+        self.h = C.c_void_p()
+        call("dtfft_hip_fft_create", int(fft_rank), int(fft_type), int(precision), int(idist), int(odist),
+             int(how_many), _i32arr(fft_sizes), _i32arr(inembed or fft_sizes), _i32arr(onembed or fft_sizes),
+             None, int(effort), C.c_void_p(stream), C.byref(self.h))
+
+    def execute(self, a, b, sign: int) -> None:
+        call("dtfft_hip_fft_execute", self.h, C.c_void_p(_ptr(a)), C.c_void_p(_ptr(b)), int(sign))
+
+    def destroy(self) -> None:
+        if self.h:
+            lib().dtfft_hip_fft_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class _Plan:
+    _create = ""
+
+    def __init__(self, dims, precision: Precision = Precision.DOUBLE, effort: Effort = Effort.ESTIMATE,
+                 executor: Executor = Executor.NONE):
+        """``dims`` in natural Fortran order (X first), as dtFFT's Python PlanC2C."""
+        self.dims = [int(d) for d in dims]
+        self.h = C.c_void_p()
+        rev = list(reversed(self.dims))
+        call(self._create, len(rev), _i32arr(rev), int(precision), int(effort), int(executor), C.byref(self.h))
+
+    @property
+    def local_sizes(self):
+        nd = len(self.dims)
+        a, b, c, d = (C.c_int32 * nd)(), (C.c_int32 * nd)(), (C.c_int32 * nd)(), (C.c_int32 * nd)()
+        sz = C.c_size_t()
+        call("dtfft_hip_get_local_sizes", self.h, a, b, c, d, C.byref(sz))
+        # C API reports reversed order; return natural Fortran order like dtFFT's Python
+        return (list(a)[::-1], list(b)[::-1], list(c)[::-1], list(d)[::-1], int(sz.value))
+
+    def _get(self, fn) -> int:
+        v = C.c_size_t()
+        call(fn, self.h, C.byref(v))
+        return int(v.value)
+
+    @property
+    def alloc_size(self) -> int:
+        return self._get("dtfft_hip_get_alloc_size")
+
+    @property
+    def element_size(self) -> int:
+        return self._get("dtfft_hip_get_element_size")
+
+    @property
+    def alloc_bytes(self) -> int:
+        return self._get("dtfft_hip_get_alloc_bytes")
+
+    @property
+    def aux_bytes(self) -> int:
+        return self._get("dtfft_hip_get_aux_bytes")
+
+    @property
+    def z_slab_enabled(self) -> bool:
+        v = C.c_int32()
+        call("dtfft_hip_get_z_slab_enabled", self.h, C.byref(v))
+        return bool(v.value)
+
+    def execute(self, inp, out, execute_type: Execute, aux=None) -> None:
+        call("dtfft_hip_execute", self.h, C.c_void_p(_ptr(inp)), C.c_void_p(_ptr(out)), int(execute_type),
+             C.c_void_p(_ptr(aux)))
+
+    def execute_stage(self, inp, out, execute_type: Execute, stage: int, aux=None) -> int:
+        """Run launch `stage` of the execute sequence only (profiling); returns the
+        number of launches in the sequence (stage < 0 runs nothing)."""
+        n = C.c_int32()
+        call("dtfft_hip_plan_execute_stage", self.h, C.c_void_p(_ptr(inp)), C.c_void_p(_ptr(out)),
+             int(execute_type), C.c_void_p(_ptr(aux)), int(stage), C.byref(n))
+        return int(n.value)
+
+    def transpose(self, inp, out, transpose_type: Transpose, aux=None) -> None:
+        call("dtfft_hip_transpose", self.h, C.c_void_p(_ptr(inp)), C.c_void_p(_ptr(out)), int(transpose_type),
+             C.c_void_p(_ptr(aux)))
+
+    def mem_alloc(self, nbytes: int | None = None) -> DeviceArray:
+        return DeviceArray(nbytes or self.alloc_bytes)
+
+    def report(self) -> None:
+        call("dtfft_hip_report", self.h)
+
+    def destroy(self) -> None:
+        if self.h:
+            lib().dtfft_hip_destroy(C.byref(self.h))
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class PlanC2C(_Plan):
+    """Complex-to-complex plan (dtfft_create_plan_c2c for one rank)."""
+    _create = "dtfft_hip_create_plan_c2c"
+
+
+class PlanR2C(_Plan):
+    """Real-to-complex plan (dtfft_create_plan_r2c for one rank)."""
+    _create = "dtfft_hip_create_plan_r2c"
+
+
+class Cycle:
+    """Batched fwd+bwd cycle over many 3D C2C grids (dims per grid in Fortran order)."""
+
+    def __init__(self, grids, precision: Precision = Precision.SINGLE, effort: Effort = Effort.ESTIMATE,
+                 stream: int | None = None):
+        self.grids = [[int(d) for d in g] for g in grids]
+        flat = [d for g in self.grids for d in reversed(g)]
+        self.h = C.c_void_p()
+        call("dtfft_hip_cycle_create", len(self.grids), _i32arr(flat), int(precision), int(effort),
+             C.c_void_p(stream), C.byref(self.h))
+
+    def layout(self):
+        offs = (C.c_int64 * len(self.grids))()
+        sz = C.c_size_t()
+        call("dtfft_hip_cycle_get_layout", self.h, offs, C.byref(sz))
+        return list(offs), int(sz.value)
+
+    def execute(self, inp, out, aux, execute_type: Execute) -> None:
+        call("dtfft_hip_cycle_execute", self.h, C.c_void_p(_ptr(inp)), C.c_void_p(_ptr(out)),
+             C.c_void_p(_ptr(aux)), int(execute_type))
+
+    def execute_graph(self, inp, out, aux, in2) -> None:
+        call("dtfft_hip_cycle_execute_graph", self.h, C.c_void_p(_ptr(inp)), C.c_void_p(_ptr(out)),
+             C.c_void_p(_ptr(aux)), C.c_void_p(_ptr(in2)))
+
+    def destroy(self) -> None:
+        if self.h:
+            lib().dtfft_hip_cycle_destroy(C.byref(self.h))
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

@@ -1,0 +1,491 @@
+// Batched 1D C2C Stockham FFT kernels (register butterflies, LDS exchange).
+//
+// One thread block transforms LPB lines of length N.  The N-point transform
+// is a sequence of radix-R passes (R in 2,3,4,5,7,8,11,13,16); pass p with span
+// L = R_0*...*R_{p-1} reads x[j + r*N/R], multiplies by w_{L*R}^{r*(j%L)},
+// runs an R-point DFT in registers and writes y[(j - j%L)*R + j%L + s*L].
+// The first pass reads global memory directly and the last pass writes it
+// directly; the passes in between exchange through LDS.
+//
+// Lines are addressed as (l, b): element n of line (l, b) sits at
+//     base + l*ls + b*bs + n*ns
+// so the same kernel covers the contiguous executor layout (ns = 1) and the
+// "transpose fused into the next FFT" layout where the FFT axis is strided and
+// consecutive lines are contiguous (ls = 1).  IN_COLS / OUT_COLS select which
+// thread mapping makes the global access coalesced: along n (rows) or along l
+// (columns).
+//
+// Forward transforms use exp(-2*pi*i/N); backward ones conjugate on load and
+// store (IDFT(x) = conj(DFT(conj(x)))), both unnormalised as in FFTW.
+#include "common.hip"
+
+namespace dtfft {
+
+// ---------------------------------------------------------------------------
+// Small DFT butterflies, forward sign, in place on v[0..R-1].
+// ---------------------------------------------------------------------------
+template <typename T, int R> struct bfly;
+
+template <typename T> struct bfly<T, 2> {
+    static __device__ __forceinline__ void run(cx<T>* v) {
+        cx<T> a = v[0] + v[1], b = v[0] - v[1];
+        v[0] = a; v[1] = b;
+    }
+};
+
+template <typename T> struct bfly<T, 4> {
+    static __device__ __forceinline__ void run(cx<T>* v) {
+        cx<T> s0 = v[0] + v[2], d0 = v[0] - v[2];
+        cx<T> s1 = v[1] + v[3], d1 = mul_mi(v[1] - v[3]);
+        v[0] = s0 + s1; v[2] = s0 - s1;
+        v[1] = d0 + d1; v[3] = d0 - d1;
+    }
+};
+
+template <typename T> struct bfly<T, 8> {
+    static __device__ __forceinline__ void run(cx<T>* v) {
+        const T h = (T)0.7071067811865476;
+        cx<T> a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) { a[i] = v[i] + v[i + 4]; b[i] = v[i] - v[i + 4]; }
+        b[1] = cmulc(b[1], h, -h);
+        b[2] = mul_mi(b[2]);
+        b[3] = cmulc(b[3], -h, -h);
+        bfly<T, 4>::run(a);
+        bfly<T, 4>::run(b);
+#pragma unroll
+        for (int i = 0; i < 4; i++) { v[2 * i] = a[i]; v[2 * i + 1] = b[i]; }
+    }
+};
+
+template <typename T> struct bfly<T, 16> {
+    static __device__ __forceinline__ void run(cx<T>* v) {
+        // 16 = 4 x 4: DFT over n2 for each n1, twiddle w16^(n1*k0), DFT over n1.
+        const T c1 = (T)0.9238795325112867, s1 = (T)0.3826834323650898, h = (T)0.7071067811865476;
+        cx<T> u[4][4];
+#pragma unroll
+        for (int n1 = 0; n1 < 4; n1++) {
+            cx<T> t[4] = {v[n1], v[n1 + 4], v[n1 + 8], v[n1 + 12]};
+            bfly<T, 4>::run(t);
+#pragma unroll
+            for (int k0 = 0; k0 < 4; k0++) u[k0][n1] = t[k0];
+        }
+        // twiddles w16^(n1*k0) = exp(-2*pi*i*n1*k0/16)
+        u[1][1] = cmulc(u[1][1], c1, -s1);
+        u[1][2] = cmulc(u[1][2], h, -h);
+        u[1][3] = cmulc(u[1][3], s1, -c1);
+        u[2][1] = cmulc(u[2][1], h, -h);
+        u[2][2] = mul_mi(u[2][2]);
+        u[2][3] = cmulc(u[2][3], -h, -h);
+        u[3][1] = cmulc(u[3][1], s1, -c1);
+        u[3][2] = cmulc(u[3][2], -h, -h);
+        u[3][3] = cmulc(u[3][3], -c1, s1);
+#pragma unroll
+        for (int k0 = 0; k0 < 4; k0++) {
+            bfly<T, 4>::run(u[k0]);
+#pragma unroll
+            for (int k1 = 0; k1 < 4; k1++) v[k0 + 4 * k1] = u[k0][k1];
+        }
+    }
+};
+
+// Odd prime radices: y_0 = sum x_r; for s = 1..(p-1)/2
+//   a_s = x_0 + sum_r (x_r + x_{p-r}) cos(2 pi r s / p)
+//   b_s = sum_r (x_r - x_{p-r}) sin(2 pi r s / p)
+//   y_s = a_s - i b_s, y_{p-s} = a_s + i b_s.
+template <int P> struct prime_consts;
+template <> struct prime_consts<3> {
+    static __device__ __forceinline__ double c(int k) { return -0.5; }
+    static __device__ __forceinline__ double s(int k) { return 0.8660254037844386; }
+};
+template <> struct prime_consts<5> {
+    static __device__ __forceinline__ double c(int k) {
+        return k == 1 ? 0.30901699437494745 : -0.8090169943749475;
+    }
+    static __device__ __forceinline__ double s(int k) {
+        return k == 1 ? 0.9510565162951535 : 0.5877852522924731;
+    }
+};
+template <> struct prime_consts<7> {
+    static __device__ __forceinline__ double c(int k) {
+        return k == 1 ? 0.6234898018587335 : (k == 2 ? -0.2225209339563144 : -0.9009688679024191);
+    }
+    static __device__ __forceinline__ double s(int k) {
+        return k == 1 ? 0.7818314824680298 : (k == 2 ? 0.9749279121818236 : 0.4338837391175581);
+    }
+};
+template <> struct prime_consts<11> {
+    static __device__ __forceinline__ double c(int k) {
+        const double t[5] = {0.8412535328311812, 0.41541501300188644, -0.14231483827328514,
+                             -0.6548607339452851, -0.9594929736144974};
+        return t[k - 1];
+    }
+    static __device__ __forceinline__ double s(int k) {
+        const double t[5] = {0.5406408174555976, 0.9096319953545183, 0.9898214418809327,
+                             0.7557495743542583, 0.28173255684142967};
+        return t[k - 1];
+    }
+};
+template <> struct prime_consts<13> {
+    static __device__ __forceinline__ double c(int k) {
+        const double t[6] = {0.8854560256532099, 0.5680647467311558, 0.12053668025532305,
+                             -0.3546048870425356, -0.7485107481711011, -0.970941817426052};
+        return t[k - 1];
+    }
+    static __device__ __forceinline__ double s(int k) {
+        const double t[6] = {0.46472317204376856, 0.8229838658936564, 0.992708874098054,
+                             0.9350162426854148, 0.6631226582407952, 0.23931566428755777};
+        return t[k - 1];
+    }
+};
+
+template <typename T, int P> struct bfly_prime {
+    static __device__ __forceinline__ void run(cx<T>* v) {
+        constexpr int H = (P - 1) / 2;
+        cx<T> sp[H], sm[H];
+        cx<T> y0 = v[0];
+#pragma unroll
+        for (int r = 1; r <= H; r++) {
+            sp[r - 1] = v[r] + v[P - r];
+            sm[r - 1] = v[r] - v[P - r];
+            y0 = y0 + sp[r - 1];
+        }
+        cx<T> out[P];
+        out[0] = y0;
+#pragma unroll
+        for (int s = 1; s <= H; s++) {
+            cx<T> a = v[0], b = mk((T)0, (T)0);
+#pragma unroll
+            for (int r = 1; r <= H; r++) {
+                const int q = (r * s) % P;  // cos/sin(2 pi q / P), folded into 1..H
+                const int qq = q <= H ? q : P - q;
+                const T c = (T)prime_consts<P>::c(qq);
+                const T sn = (T)(q <= H ? prime_consts<P>::s(qq) : -prime_consts<P>::s(qq));
+                a = a + scale(sp[r - 1], c);
+                b = b + scale(sm[r - 1], sn);
+            }
+            // y_s = a - i b ; y_{P-s} = a + i b
+            out[s] = mk(a.x + b.y, a.y - b.x);
+            out[P - s] = mk(a.x - b.y, a.y + b.x);
+        }
+#pragma unroll
+        for (int i = 0; i < P; i++) v[i] = out[i];
+    }
+};
+template <typename T> struct bfly<T, 3> : bfly_prime<T, 3> {};
+template <typename T> struct bfly<T, 5> : bfly_prime<T, 5> {};
+template <typename T> struct bfly<T, 7> : bfly_prime<T, 7> {};
+template <typename T> struct bfly<T, 11> : bfly_prime<T, 11> {};
+template <typename T> struct bfly<T, 13> : bfly_prime<T, 13> {};
+
+// ---------------------------------------------------------------------------
+// Radix schedule (compile time).
+// ---------------------------------------------------------------------------
+template <int N_, int R0, int R1 = 1, int R2 = 1, int R3 = 1, int R4 = 1, int R5 = 1, int R6 = 1>
+struct sched {
+    static constexpr int N = N_;
+    static constexpr int NP = (R0 > 1) + (R1 > 1) + (R2 > 1) + (R3 > 1) + (R4 > 1) + (R5 > 1) + (R6 > 1);
+    static constexpr __device__ __host__ int radix(int p) {
+        return p == 0 ? R0 : p == 1 ? R1 : p == 2 ? R2 : p == 3 ? R3 : p == 4 ? R4 : p == 5 ? R5 : R6;
+    }
+    static constexpr __device__ __host__ int span(int p) {  // L_p = product of radices before p
+        return p == 0 ? 1 : span(p - 1) * radix(p - 1);
+    }
+    // offset of pass p's twiddles in the table: sum over q < p of L_q * (R_q - 1)
+    static constexpr __device__ __host__ int tw_off(int p) {
+        return p == 0 ? 0 : tw_off(p - 1) + span(p - 1) * (radix(p - 1) - 1);
+    }
+    static constexpr int TW_SIZE = tw_off(NP);
+};
+
+// Launch geometry of one kernel: (l, b) line space and strides in elements.
+struct lines_desc {
+    i64 nl, nb;          // lines per row, rows
+    i64 in_ls, in_bs, in_ns;
+    i64 out_ls, out_bs, out_ns;
+    i64 ntl;             // ceil(nl / LPB)
+    i64 ntiles;          // ntl * nb
+};
+
+template <typename T, class S, int TPL, int LPB, bool IN_COLS, bool OUT_COLS, int PAD>
+struct fft_tile {
+    typedef typename tw_of<T>::type tw_t;
+    static constexpr int N = S::N;
+    static constexpr int NP = S::NP;
+    static constexpr int LD = N + PAD;  // LDS line stride (complex elements)
+    static constexpr int LDS_ELEMS = LPB * LD;
+    static constexpr int NT = TPL * LPB;
+
+    // thread -> (line, j) mapping: rows (j fastest) or columns (line fastest)
+    template <bool COLS>
+    static __device__ __forceinline__ void map(int tid, int& line, int& j) {
+        if (COLS) { line = tid % LPB; j = tid / LPB; }
+        else { line = tid / TPL; j = tid % TPL; }
+    }
+
+    // One radix pass.  SRC_G: read global (first pass); DST_G: write global (last pass);
+    // otherwise the pass reads / writes the line in LDS (in place, with a barrier between).
+    template <int p, bool SRC_G, bool DST_G, bool COLS>
+    static __device__ __forceinline__ void pass(const cx<T>* __restrict__ gin, cx<T>* __restrict__ gout,
+                                                const tw_t* __restrict__ tw, cx<T>* lds,
+                                                i64 ibase, i64 obase, const lines_desc& d, i64 l0,
+                                                bool inv) {
+        constexpr int R = S::radix(p);
+        constexpr int L = S::span(p);
+        constexpr int NB = N / R;
+        constexpr int IT = (NB + TPL - 1) / TPL;
+        constexpr bool EXACT = (IT * TPL == NB);
+        const int tid = threadIdx.x;
+        int line, j;
+        map<COLS>(tid, line, j);
+        const bool lv = (l0 + line) < d.nl;
+        cx<T> v[IT][R];
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int b = j + it * TPL;
+            if (EXACT || b < NB) {
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int n = b + r * NB;
+                    if (SRC_G) {
+                        cx<T> x = mk((T)0, (T)0);
+                        if (lv) x = gin[ibase + (i64)line * d.in_ls + (i64)n * d.in_ns];
+                        v[it][r] = inv ? conj(x) : x;
+                    } else {
+                        v[it][r] = lds[line * LD + n];
+                    }
+                }
+            }
+        }
+        if (!SRC_G && !DST_G) __syncthreads();  // everyone has read before the in-place write
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int b = j + it * TPL;
+            if (EXACT || b < NB) {
+                const int k = b % L;
+                if (L > 1) {
+#pragma unroll
+                    for (int r = 1; r < R; r++)
+                        v[it][r] = twmul(v[it][r], tw[S::tw_off(p) + k * (R - 1) + (r - 1)]);
+                }
+                bfly<T, R>::run(v[it]);
+                const int base = (b - k) * R + k;
+#pragma unroll
+                for (int s = 0; s < R; s++) {
+                    const int n = base + s * L;
+                    if (DST_G) {
+                        if (lv) {
+                            cx<T> y = v[it][s];
+                            gout[obase + (i64)line * d.out_ls + (i64)n * d.out_ns] = inv ? conj(y) : y;
+                        }
+                    } else {
+                        lds[line * LD + n] = v[it][s];
+                    }
+                }
+            }
+        }
+    }
+
+    // Runs passes p..NP-1.  FIRST_G: pass 0 reads global (else LDS was filled and synced by
+    // the caller).  LAST_G: the last pass writes global (else it leaves the line in LDS;
+    // the caller syncs before reading it).
+    template <int p, bool FIRST_G, bool LAST_G>
+    static __device__ __forceinline__ void passes(const cx<T>* __restrict__ gin, cx<T>* __restrict__ gout,
+                                                  const tw_t* __restrict__ tw, cx<T>* lds, i64 ibase,
+                                                  i64 obase, const lines_desc& d, i64 l0, bool inv) {
+        if constexpr (NP == 0 && p == 0 && FIRST_G != LAST_G) {
+            // length-1 transform: only move the line between global memory and LDS
+            for (int line = threadIdx.x; line < LPB; line += NT) {
+                if (l0 + line >= d.nl) continue;
+                if (FIRST_G) {
+                    cx<T> x = gin[ibase + (i64)line * d.in_ls];
+                    lds[line * LD] = inv ? conj(x) : x;
+                } else {
+                    cx<T> y = lds[line * LD];
+                    gout[obase + (i64)line * d.out_ls] = inv ? conj(y) : y;
+                }
+            }
+        }
+        if constexpr (p < NP) {
+            constexpr bool first = (p == 0);
+            constexpr bool last = (p == NP - 1);
+            if constexpr (first && FIRST_G && last) {
+                // single pass from global: stage the input through LDS so that in-place use
+                // is safe and the read and write mappings may differ
+                for (int e = threadIdx.x; e < LPB * N; e += NT) {
+                    int line, n;
+                    if (IN_COLS) { line = e % LPB; n = e / LPB; } else { line = e / N; n = e % N; }
+                    cx<T> x = mk((T)0, (T)0);
+                    if (l0 + line < d.nl) x = gin[ibase + (i64)line * d.in_ls + (i64)n * d.in_ns];
+                    lds[line * LD + n] = inv ? conj(x) : x;
+                }
+                __syncthreads();
+                pass<p, false, LAST_G, OUT_COLS>(gin, gout, tw, lds, ibase, obase, d, l0, inv);
+            } else if constexpr (first && FIRST_G) {
+                pass<p, true, false, IN_COLS>(gin, gout, tw, lds, ibase, obase, d, l0, inv);
+            } else if constexpr (first) {
+                pass<p, false, last && LAST_G, (last && LAST_G) ? OUT_COLS : false>(gin, gout, tw, lds, ibase,
+                                                                                   obase, d, l0, inv);
+            } else if constexpr (last) {
+                __syncthreads();
+                pass<p, false, LAST_G, LAST_G ? OUT_COLS : false>(gin, gout, tw, lds, ibase, obase, d, l0, inv);
+            } else {
+                __syncthreads();
+                pass<p, false, false, false>(gin, gout, tw, lds, ibase, obase, d, l0, inv);
+            }
+            passes<p + 1, FIRST_G, LAST_G>(gin, gout, tw, lds, ibase, obase, d, l0, inv);
+        }
+    }
+
+    static __device__ __forceinline__ void tile_bases(const lines_desc& d, i64 tile, i64& l0, i64& ibase,
+                                                      i64& obase) {
+        const i64 tl = tile % d.ntl;
+        const i64 b = tile / d.ntl;
+        l0 = tl * LPB;
+        ibase = l0 * d.in_ls + b * d.in_bs;
+        obase = l0 * d.out_ls + b * d.out_bs;
+    }
+
+    static __device__ __forceinline__ void run(const cx<T>* __restrict__ gin, cx<T>* __restrict__ gout,
+                                               const tw_t* __restrict__ tw, cx<T>* lds,
+                                               const lines_desc& d, i64 tile, bool inv) {
+        i64 l0, ibase, obase;
+        tile_bases(d, tile, l0, ibase, obase);
+        if constexpr (NP == 0) {
+            return;
+        } else {
+            passes<0, true, true>(gin, gout, tw, lds, ibase, obase, d, l0, inv);
+        }
+    }
+};
+
+}  // namespace dtfft
+
+// Plain batched C2C launch: one tile per block (grid-stride over tiles).
+template <typename T, class S, int TPL, int LPB, bool IN_COLS, bool OUT_COLS, int PAD>
+__global__ void __launch_bounds__(TPL * LPB)
+fft_c2c_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
+               const typename tw_of<T>::type* __restrict__ tw, dtfft::lines_desc d, int inv) {
+    typedef dtfft::fft_tile<T, S, TPL, LPB, IN_COLS, OUT_COLS, PAD> F;
+    __shared__ cx<T> lds[F::LDS_ELEMS];
+    for (i64 t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
+        F::run(in, out, tw, lds, d, t, inv != 0);
+        __syncthreads();
+    }
+}
+
+// Real-to-complex along contiguous lines of even length n = 2M (rows layout).
+// The line is read as M complex values z_k = x_2k + i x_2k+1, transformed with the
+// M-point schedule S, and untangled:
+//   X_k = ((Z_k + conj Z_{M-k}) - i W^k (Z_k - conj Z_{M-k})) / 2,  W = exp(-2 pi i / n),
+// for k = 0..M (Z_M = Z_0).  Strides in `d` are in complex units of the input view
+// (in_ls = n/2) and of the output (out_ls = odist).  rtw holds W^k, k = 0..M.
+template <typename T, class S, int TPL, int LPB, int PAD>
+__global__ void __launch_bounds__(TPL * LPB)
+fft_r2c_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
+               const typename tw_of<T>::type* __restrict__ tw,
+               const typename tw_of<T>::type* __restrict__ rtw, dtfft::lines_desc d) {
+    typedef dtfft::fft_tile<T, S, TPL, LPB, false, false, PAD> F;
+    constexpr int M = S::N;
+    __shared__ cx<T> lds[F::LDS_ELEMS];
+    for (i64 t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
+        i64 l0, ibase, obase;
+        F::tile_bases(d, t, l0, ibase, obase);
+        F::template passes<0, true, false>(in, out, tw, lds, ibase, obase, d, l0, false);
+        __syncthreads();
+        const T half = (T)0.5;
+        for (int e = threadIdx.x; e < LPB * (M + 1); e += F::NT) {
+            const int line = e / (M + 1), k = e % (M + 1);
+            if (l0 + line >= d.nl) continue;
+            const cx<T> a = lds[line * F::LD + (k == M ? 0 : k)];
+            const cx<T> b = conj(lds[line * F::LD + (k == 0 ? 0 : M - k)]);
+            const cx<T> s = a + b, df = a - b;
+            const cx<T> w = twmul(df, rtw[k]);           // W^k (Z_k - conj Z_{M-k})
+            const cx<T> x = mk(half * (s.x + w.y), half * (s.y - w.x));  // (s - i w) / 2
+            out[obase + (i64)line * d.out_ls + k] = x;
+        }
+        __syncthreads();
+    }
+}
+
+// Complex-to-real inverse of fft_r2c_kernel (FFTW c2r semantics: Im X_0 and Im X_M are
+// ignored):  2E_k = X_k + conj X_{M-k},  2O_k = (X_k - conj X_{M-k}) conj(W^k),
+// Z_k = 2E_k + i 2O_k, k < M; the backward M-point transform of Z gives
+// y_2j + i y_2j+1.  Input strides in complex units (in_ls = idist), output strides in
+// complex units of the real output view (out_ls = n/2).
+template <typename T, class S, int TPL, int LPB, int PAD>
+__global__ void __launch_bounds__(TPL * LPB)
+fft_c2r_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
+               const typename tw_of<T>::type* __restrict__ tw,
+               const typename tw_of<T>::type* __restrict__ rtw, dtfft::lines_desc d) {
+    typedef dtfft::fft_tile<T, S, TPL, LPB, false, false, PAD> F;
+    constexpr int M = S::N;
+    __shared__ cx<T> lds[F::LDS_ELEMS];
+    for (i64 t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
+        i64 l0, ibase, obase;
+        F::tile_bases(d, t, l0, ibase, obase);
+        for (int e = threadIdx.x; e < LPB * M; e += F::NT) {
+            const int line = e / M, k = e % M;
+            cx<T> z = mk((T)0, (T)0);
+            if (l0 + line < d.nl) {
+                const i64 lb = ibase + (i64)line * d.in_ls;
+                cx<T> a = in[lb + k];
+                cx<T> b = in[lb + (M - k)];
+                if (k == 0) { a.y = (T)0; b.y = (T)0; }  // Im X_0, Im X_M do not contribute
+                b = conj(b);
+                const cx<T> e2 = a + b;
+                const cx<T> o2 = twmul(a - b, conj_tw(rtw[k]));
+                z = mk(e2.x - o2.y, e2.y + o2.x);  // 2E + i 2O
+            }
+            // backward transform = conj(forward(conj z)): store conj(z); the last pass
+            // conjugates on its global write (inv = true)
+            lds[line * F::LD + k] = conj(z);
+        }
+        __syncthreads();
+        F::template passes<0, false, true>(in, out, tw, lds, ibase, obase, d, l0, true);
+        __syncthreads();
+    }
+}
+
+// Twiddle table of one pass: tw[off + k*(R-1) + (r-1)] = exp(-2*pi*i*r*k/(L*R)),
+// generated in FP64 (sincospi) and stored as FP32 hi+lo pairs or FP64.
+__global__ void gen_twiddles_f32(twf* tw, int off, int L, int R) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= L * (R - 1)) return;
+    const int k = e / (R - 1), r = e % (R - 1) + 1;
+    double s, c;
+    sincospi(-2.0 * (double)(r * k) / (double)(L * R), &s, &c);
+    twf w;
+    w.rh = (float)c; w.rl = (float)(c - (double)w.rh);
+    w.ih = (float)s; w.il = (float)(s - (double)w.ih);
+    tw[off + e] = w;
+}
+
+__global__ void gen_twiddles_f64(cx<double>* tw, int off, int L, int R) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= L * (R - 1)) return;
+    const int k = e / (R - 1), r = e % (R - 1) + 1;
+    double s, c;
+    sincospi(-2.0 * (double)(r * k) / (double)(L * R), &s, &c);
+    tw[off + e] = mk(c, s);
+}
+
+// R2C / C2R untangling twiddles: tw[off + k] = exp(-2*pi*i*k/n), k < cnt.
+__global__ void gen_rtwiddles_f32(twf* tw, int off, int n, int cnt) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= cnt) return;
+    double s, c;
+    sincospi(-2.0 * (double)k / (double)n, &s, &c);
+    twf w;
+    w.rh = (float)c; w.rl = (float)(c - (double)w.rh);
+    w.ih = (float)s; w.il = (float)(s - (double)w.ih);
+    tw[off + k] = w;
+}
+
+__global__ void gen_rtwiddles_f64(cx<double>* tw, int off, int n, int cnt) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= cnt) return;
+    double s, c;
+    sincospi(-2.0 * (double)k / (double)n, &s, &c);
+    tw[off + k] = mk(c, s);
+}

@@ -1,0 +1,433 @@
+// FFT executor plugin: batched 1D / 2D C2C (and R2C / C2R) stages on gfx950.
+//
+// Drop-in for the cuFFT / VkFFT executors behind abstract_executor
+// (src/dtfft_abstract_executor.F90:47-112; cuFFT: src/interfaces/fft/cufft/
+// dtfft_executor_cufft_m.F90:52-125).  create receives exactly what
+// abstract_executor%create computes from the pencils (:115-216): fft_rank, the sizes
+// in FFTW row-major order, inembed/onembed, idist/odist and how_many, unit stride.
+//
+// Each 1D stage is one launch of the Stockham kernel in device/fft.hip, compiled by
+// hipRTC for its length.  Radix schedule, threads per line, lines per block and LDS
+// padding come from a heuristic or, at effort >= MEASURE, from timing the candidates
+// on the device (the project's replacement for dtfft_nvrtc_block_optimizer).
+#include <algorithm>
+#include <array>
+#include <cmath>
+
+#include "internal.h"
+
+namespace dtfft_hip {
+
+// ---- schedules ----------------------------------------------------------------------
+bool factorize(int n, std::vector<int>& radices) {
+    radices.clear();
+    if (n < 1) return false;
+    if (n == 1) { radices.push_back(1); return true; }
+    int a = 0, m = n;
+    while (m % 2 == 0) { m /= 2; a++; }
+    // power-of-two part: fewest passes with radices 16/8/4/2
+    if (a > 0) {
+        int passes = (a + 3) / 4;
+        std::vector<int> e(passes, a / passes);
+        for (int i = 0; i < a % passes; i++) e[i]++;
+        for (int x : e) radices.push_back(1 << x);
+    }
+    const int primes[] = {3, 5, 7, 11, 13};
+    for (int p : primes)
+        while (m % p == 0) { radices.push_back(p); m /= p; }
+    if (m != 1) return false;
+    if (radices.size() > 7) return false;
+    return true;
+}
+
+std::string fft_config::expr() const {
+    std::string s = kind == 0 ? "fft_c2c_kernel<" : (kind == 1 ? "fft_r2c_kernel<" : "fft_c2r_kernel<");
+    s += dbl ? "double" : "float";
+    s += ", dtfft::sched<" + std::to_string(n);
+    for (int r : radices) s += ", " + std::to_string(r);
+    s += ">, " + std::to_string(tpl) + ", " + std::to_string(lpb) + ", ";
+    if (kind == 0) s += std::string(in_cols ? "true" : "false") + ", " + (out_cols ? "true" : "false") + ", ";
+    s += std::to_string(pad) + ">";
+    return s;
+}
+
+static int elem_bytes(bool dbl) { return dbl ? 16 : 8; }
+static const size_t kMaxLds = 160 * 1024;
+
+static bool valid(const fft_config& c) {
+    const int threads = c.tpl * c.lpb;
+    if (threads < 1 || threads > 1024) return false;
+    if ((size_t)c.lpb * (c.n + c.pad) * elem_bytes(c.dbl) > kMaxLds) return false;
+    return true;
+}
+
+fft_config default_fft_config(int n, bool dbl, bool in_cols, bool out_cols, bool inplace) {
+    (void)inplace;
+    fft_config c;
+    c.n = n; c.dbl = dbl; c.in_cols = in_cols; c.out_cols = out_cols;
+    factorize(n, c.radices);
+    int rmax = *std::max_element(c.radices.begin(), c.radices.end());
+    c.tpl = std::max(1, n / rmax);
+    if (in_cols || out_cols) {
+        c.pad = 1;
+        c.lpb = dbl ? 8 : 16;
+        while (c.tpl * c.lpb > 1024 && c.tpl > 1) c.tpl = (c.tpl + 1) / 2;
+        while (!valid(c) && c.lpb > 1) c.lpb /= 2;
+    } else {
+        c.pad = 0;
+        c.lpb = std::max(1, 256 / c.tpl);
+        while ((size_t)c.lpb * n * elem_bytes(dbl) > 64 * 1024 && c.lpb > 1) c.lpb /= 2;
+        while (c.tpl * c.lpb > 1024 && c.tpl > 1) c.tpl = (c.tpl + 1) / 2;
+    }
+    return c;
+}
+
+std::vector<fft_config> fft_candidates(int n, bool dbl, bool in_cols, bool out_cols, bool inplace) {
+    std::vector<fft_config> out;
+    fft_config base = default_fft_config(n, dbl, in_cols, out_cols, inplace);
+    out.push_back(base);
+    std::vector<int> tpls, lpbs;
+    for (int t = base.tpl; t >= 1 && (int)tpls.size() < 3; t /= 2) tpls.push_back(t);
+    if (in_cols || out_cols) lpbs = {4, 8, 16, 32};
+    else lpbs = {1, 2, 4, 8, 16, 32, 64};
+    for (int t : tpls)
+        for (int l : lpbs)
+            for (int p : {0, 1}) {
+                fft_config c = base;
+                c.tpl = t; c.lpb = l; c.pad = p;
+                if (!valid(c)) continue;
+                if (c.tpl * c.lpb < 64) continue;
+                bool dup = false;
+                for (auto& o : out) dup |= (o.tpl == c.tpl && o.lpb == c.lpb && o.pad == c.pad);
+                if (!dup) out.push_back(c);
+            }
+    return out;
+}
+
+// ---- twiddles -------------------------------------------------------------------------
+twiddles::~twiddles() {
+    // Tables live in a process-wide cache and are released at exit; freeing device memory
+    // from a static destructor after the runtime shut down is not safe, so leave it.
+}
+
+static std::mutex g_tw_mu;
+static std::map<std::string, std::shared_ptr<twiddles>> g_tw;
+
+int32_t get_twiddles(int n, const std::vector<int>& radices, bool dbl, std::shared_ptr<twiddles>* out) {
+    std::string key = std::to_string(n) + (dbl ? "d" : "f");
+    for (int r : radices) key += "," + std::to_string(r);
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    auto it = g_tw.find(key);
+    if (it != g_tw.end()) { *out = it->second; return DTFFT_HIP_SUCCESS; }
+    auto t = std::make_shared<twiddles>();
+    int64_t total = 0, L = 1;
+    std::vector<std::array<int64_t, 3>> passes;  // (off, L, R)
+    for (int r : radices) {
+        passes.push_back({total, L, r});
+        total += L * (r - 1);
+        L *= r;
+    }
+    const rtc_kernel* k = nullptr;
+    TRY(rtc_get("fft", dbl ? "gen_twiddles_f64" : "gen_twiddles_f32", &k));
+    t->bytes = std::max<int64_t>(total, 1) * 16;
+    if (!compile_only()) {
+        HIP_TRY(hipMalloc(&t->ptr, (size_t)t->bytes));
+        for (auto& p : passes) {
+            int off = (int)p[0], Lp = (int)p[1], R = (int)p[2];
+            int cnt = Lp * (R - 1);
+            if (cnt <= 0) continue;
+            void* ptr = t->ptr;
+            void* args[] = {&ptr, &off, &Lp, &R};
+            TRY(launch(k, (unsigned)((cnt + 255) / 256), 256, nullptr, args));
+        }
+        HIP_TRY(hipDeviceSynchronize());
+    }
+    g_tw[key] = t;
+    *out = t;
+    return DTFFT_HIP_SUCCESS;
+}
+
+int32_t get_rtwiddles(int n, bool dbl, std::shared_ptr<twiddles>* out) {
+    std::string key = "r" + std::to_string(n) + (dbl ? "d" : "f");
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    auto it = g_tw.find(key);
+    if (it != g_tw.end()) { *out = it->second; return DTFFT_HIP_SUCCESS; }
+    auto t = std::make_shared<twiddles>();
+    int cnt = n / 2 + 1;
+    const rtc_kernel* k = nullptr;
+    TRY(rtc_get("fft", dbl ? "gen_rtwiddles_f64" : "gen_rtwiddles_f32", &k));
+    t->bytes = (int64_t)cnt * 16;
+    if (!compile_only()) {
+        HIP_TRY(hipMalloc(&t->ptr, (size_t)t->bytes));
+        void* ptr = t->ptr;
+        int off = 0, nn = n;
+        void* args[] = {&ptr, &off, &nn, &cnt};
+        TRY(launch(k, (unsigned)((cnt + 255) / 256), 256, nullptr, args));
+        HIP_TRY(hipDeviceSynchronize());
+    }
+    g_tw[key] = t;
+    *out = t;
+    return DTFFT_HIP_SUCCESS;
+}
+
+// ---- stages ---------------------------------------------------------------------------
+lines_desc make_lines(int64_t nl, int64_t nb, int64_t in_ls, int64_t in_bs, int64_t in_ns, int64_t out_ls,
+                      int64_t out_bs, int64_t out_ns, int lpb) {
+    lines_desc d;
+    d.nl = nl; d.nb = nb;
+    d.in_ls = in_ls; d.in_bs = in_bs; d.in_ns = in_ns;
+    d.out_ls = out_ls; d.out_bs = out_bs; d.out_ns = out_ns;
+    d.ntl = (nl + lpb - 1) / lpb;
+    d.ntiles = d.ntl * nb;
+    return d;
+}
+
+int32_t fft_stage::init(const fft_config& c, const lines_desc& desc) {
+    cfg = c;
+    d = desc;
+    d.ntl = (d.nl + c.lpb - 1) / c.lpb;
+    d.ntiles = d.ntl * d.nb;
+    if (c.n > 1 || c.kind != 0) {
+        TRY(rtc_get("fft", c.expr(), &k));
+        TRY(get_twiddles(c.n, c.radices, c.dbl, &tw));
+    }
+    if (c.kind != 0) TRY(get_rtwiddles(2 * c.n, c.dbl, &rtw));
+    block = (unsigned)(c.tpl * c.lpb);
+    grid = (unsigned)std::min<int64_t>(d.ntiles, 1 << 22);
+    return DTFFT_HIP_SUCCESS;
+}
+
+int32_t fft_stage::run(const void* in, void* out, int sign, hipStream_t s) const {
+    if (d.ntiles == 0) return DTFFT_HIP_SUCCESS;
+    if (cfg.kind != 0) {
+        void* twp = tw ? tw->ptr : nullptr;
+        void* rtwp = rtw ? rtw->ptr : nullptr;
+        lines_desc dd = d;
+        void* args[] = {(void*)&in, &out, &twp, &rtwp, &dd};
+        return launch(k, grid, block, s, args);
+    }
+    if (cfg.n == 1) {
+        // length-1 transform is the identity (copy the lines)
+        if (in == out) return DTFFT_HIP_SUCCESS;
+        if (d.in_ls == d.out_ls && d.in_bs == d.out_bs && (d.nb == 1 || d.in_bs == d.nl * d.in_ls) && d.in_ls == 1) {
+            HIP_TRY(hipMemcpyAsync(out, in, (size_t)(d.nl * d.nb) * elem_bytes(cfg.dbl), hipMemcpyDeviceToDevice, s));
+            return DTFFT_HIP_SUCCESS;
+        }
+        set_last_error("length-1 FFT stage with a strided layout");
+        return DTFFT_HIP_ERROR_UNSUPPORTED_SIZE;
+    }
+    int inv = sign > 0 ? 1 : 0;
+    void* twp = tw ? tw->ptr : nullptr;
+    lines_desc dd = d;
+    void* args[] = {(void*)&in, &out, &twp, &dd, &inv};
+    return launch(k, grid, block, s, args);
+}
+
+static double time_stage(const fft_stage& st, void* in, void* out, hipStream_t s, int warm, int iters) {
+    for (int i = 0; i < warm; i++) if (st.run(in, out, -1, s)) return 1e30;
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess) return 1e30;
+    if (hipEventCreate(&b) != hipSuccess) { hipEventDestroy(a); return 1e30; }
+    hipEventRecord(a, s);
+    for (int i = 0; i < iters; i++) st.run(in, out, -1, s);
+    hipEventRecord(b, s);
+    hipEventSynchronize(b);
+    float ms = 0;
+    hipEventElapsedTime(&ms, a, b);
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    return ms / iters;
+}
+
+static int64_t span(const lines_desc& d, bool in, int n) {
+    int64_t ls = in ? d.in_ls : d.out_ls, bs = in ? d.in_bs : d.out_bs, ns = in ? d.in_ns : d.out_ns;
+    return (d.nl - 1) * ls + (d.nb - 1) * bs + (int64_t)(n - 1) * ns + 1;
+}
+
+int32_t plan_fft_stage(int n, bool dbl, int64_t nl, int64_t nb, int64_t in_ls, int64_t in_bs, int64_t in_ns,
+                       int64_t out_ls, int64_t out_bs, int64_t out_ns, bool inplace, int effort,
+                       hipStream_t stream, fft_stage* st) {
+    std::vector<int> rad;
+    if (!factorize(n, rad)) {
+        set_last_error("FFT length " + std::to_string(n) + " has a prime factor > 13");
+        return DTFFT_HIP_ERROR_UNSUPPORTED_SIZE;
+    }
+    const bool in_cols = (in_ns != 1 && in_ls == 1);
+    const bool out_cols = (out_ns != 1 && out_ls == 1);
+    fft_config cfg = default_fft_config(n, dbl, in_cols, out_cols, inplace);
+    lines_desc d = make_lines(nl, nb, in_ls, in_bs, in_ns, out_ls, out_bs, out_ns, cfg.lpb);
+    char kb[256];
+    std::snprintf(kb, sizeof kb, "fft|%d|%c|%d%d%d|%lld|%lld|%lld,%lld,%lld|%lld,%lld,%lld", n, dbl ? 'd' : 'f',
+                  (int)in_cols, (int)out_cols, (int)inplace, (long long)nl, (long long)nb, (long long)in_ls,
+                  (long long)in_bs, (long long)in_ns, (long long)out_ls, (long long)out_bs, (long long)out_ns);
+    const std::string key(kb);
+    std::string rec;
+    if (n > 1 && tune_lookup(key, rec)) {
+        int t, l, p;
+        if (std::sscanf(rec.c_str(), "%d %d %d", &t, &l, &p) == 3) {
+            fft_config c = cfg;
+            c.tpl = t; c.lpb = l; c.pad = p;
+            if (valid(c)) cfg = c;
+        }
+    } else if (n > 1 && effort >= DTFFT_HIP_MEASURE && !compile_only()) {
+        const int64_t si = span(d, true, n), so = span(d, false, n);
+        void *bi = nullptr, *bo = nullptr;
+        const size_t es = (size_t)elem_bytes(dbl);
+        HIP_TRY(hipMalloc(&bi, (size_t)si * es));
+        if (!inplace && hipMalloc(&bo, (size_t)so * es) != hipSuccess) { hipFree(bi); return DTFFT_HIP_ERROR_ALLOC_FAILED; }
+        if (inplace) bo = bi;
+        HIP_TRY(hipMemsetAsync(bi, 0, (size_t)si * es, stream));
+        double best = 1e30;
+        auto cands = fft_candidates(n, dbl, in_cols, out_cols, inplace);
+        if (effort < DTFFT_HIP_EXHAUSTIVE && cands.size() > 12) cands.resize(12);
+        for (auto& c : cands) {
+            fft_stage cand;
+            if (cand.init(c, d)) continue;
+            double ms = time_stage(cand, bi, bo, stream, 2, 5);
+            if (ms < best) { best = ms; cfg = c; }
+        }
+        hipStreamSynchronize(stream);
+        hipFree(bi);
+        if (!inplace) hipFree(bo);
+        char buf[64];
+        std::snprintf(buf, sizeof buf, "%d %d %d", cfg.tpl, cfg.lpb, cfg.pad);
+        tune_store(key, buf);
+    }
+    return st->init(cfg, d);
+}
+
+int32_t plan_real_stage(int kind, int n, bool dbl, int64_t how_many, int64_t idist, int64_t odist, int effort,
+                        hipStream_t stream, fft_stage* st) {
+    (void)effort; (void)stream;
+    if (n < 2 || n % 2 != 0) {
+        set_last_error("R2C/C2R stages need an even length, got " + std::to_string(n));
+        return DTFFT_HIP_ERROR_UNSUPPORTED_SIZE;
+    }
+    const int m = n / 2;
+    fft_config c;
+    if (!factorize(m, c.radices)) {
+        set_last_error("R2C length " + std::to_string(n) + ": n/2 has a prime factor > 13");
+        return DTFFT_HIP_ERROR_UNSUPPORTED_SIZE;
+    }
+    c = default_fft_config(m, dbl, false, false, false);
+    c.kind = kind;
+    if (m == 1) { c.radices = {1}; c.tpl = 1; c.lpb = 64; }
+    lines_desc d;
+    if (kind == 1) {  // real lines (idist reals) viewed as complex; complex out (odist)
+        if (idist % 2) { set_last_error("R2C idist must be even"); return DTFFT_HIP_ERROR_INVALID_ARGUMENT; }
+        d = make_lines(how_many, 1, idist / 2, 0, 1, odist, 0, 1, c.lpb);
+    } else {          // complex in (idist), real out viewed as complex (odist / 2)
+        if (odist % 2) { set_last_error("C2R odist must be even"); return DTFFT_HIP_ERROR_INVALID_ARGUMENT; }
+        d = make_lines(how_many, 1, idist, 0, 1, odist / 2, 0, 1, c.lpb);
+    }
+    return st->init(c, d);
+}
+
+// ---- executor handle -------------------------------------------------------------------
+struct fft_plan {
+    int rank = 1, type = DTFFT_HIP_FFT_C2C;
+    bool dbl = false;
+    hipStream_t stream = nullptr;
+    // C2C: rank 1 one stage; rank 2 fast axis (in -> out) then slow axis (in place on out).
+    std::vector<fft_stage> stages;
+    // R2C: forward R2C along the fast axis then C2C along the slow one (in place on out);
+    // backward C2C along the slow axis in place on the input (FFTW_DESTROY_INPUT, as the
+    // reference's FFTW flags allow) then C2R along the fast axis.
+    std::vector<fft_stage> bwd;
+};
+
+}  // namespace dtfft_hip
+
+using namespace dtfft_hip;
+
+extern "C" {
+
+int32_t dtfft_hip_fft_create(int8_t fft_rank, int8_t fft_type, int32_t precision, int32_t idist, int32_t odist,
+                             int32_t how_many, const int32_t* fft_sizes, const int32_t* inembed,
+                             const int32_t* onembed, const int32_t* r2r_kinds, int32_t effort, void* stream,
+                             void** plan) {
+    (void)inembed; (void)r2r_kinds;
+    if (!plan || !fft_sizes) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    *plan = nullptr;
+    if (fft_rank != 1 && fft_rank != 2) return DTFFT_HIP_ERROR_INVALID_N_DIMENSIONS;
+    if (precision != DTFFT_HIP_SINGLE && precision != DTFFT_HIP_DOUBLE) return DTFFT_HIP_ERROR_INVALID_PRECISION;
+    if (fft_type == DTFFT_HIP_FFT_R2R) return DTFFT_HIP_ERROR_R2R_FFT_NOT_SUPPORTED;
+    if (fft_type != DTFFT_HIP_FFT_C2C && fft_type != DTFFT_HIP_FFT_R2C) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    if (how_many < 0) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    if (effort < DTFFT_HIP_ESTIMATE || effort > DTFFT_HIP_EXHAUSTIVE) return DTFFT_HIP_ERROR_INVALID_EFFORT_FLAG;
+    for (int i = 0; i < fft_rank; i++) if (fft_sizes[i] <= 0) return DTFFT_HIP_ERROR_INVALID_DIMENSION_SIZE;
+    auto p = std::make_unique<fft_plan>();
+    p->rank = fft_rank;
+    p->type = fft_type;
+    p->dbl = precision == DTFFT_HIP_DOUBLE;
+    p->stream = (hipStream_t)stream;
+    if (fft_type == DTFFT_HIP_FFT_C2C) {
+        if (fft_rank == 1) {
+            fft_stage st;
+            TRY(plan_fft_stage(fft_sizes[0], p->dbl, how_many, 1, idist, 0, 1, odist, 0, 1, false, effort,
+                               p->stream, &st));
+            p->stages.push_back(st);
+        } else {
+            const int n0 = fft_sizes[0], n1 = fft_sizes[1];  // n0 slow, n1 fast (FFTW order)
+            fft_stage a, b;
+            TRY(plan_fft_stage(n1, p->dbl, (int64_t)n0 * how_many, 1, n1, 0, 1, n1, 0, 1, false, effort,
+                               p->stream, &a));
+            TRY(plan_fft_stage(n0, p->dbl, n1, how_many, 1, odist, n1, 1, odist, n1, true, effort, p->stream, &b));
+            p->stages.push_back(a);
+            p->stages.push_back(b);
+        }
+    } else {
+        if (fft_rank == 1) {
+            fft_stage f, b;
+            TRY(plan_real_stage(1, fft_sizes[0], p->dbl, how_many, idist, odist, effort, p->stream, &f));
+            TRY(plan_real_stage(2, fft_sizes[0], p->dbl, how_many, odist, idist, effort, p->stream, &b));
+            p->stages.push_back(f);
+            p->bwd.push_back(b);
+        } else {
+            const int n0 = fft_sizes[0], n1 = fft_sizes[1];
+            const int c1 = onembed ? onembed[1] : n1 / 2 + 1;  // complex fast extent
+            fft_stage f, fs, bs, b;
+            TRY(plan_real_stage(1, n1, p->dbl, (int64_t)n0 * how_many, n1, c1, effort, p->stream, &f));
+            TRY(plan_fft_stage(n0, p->dbl, c1, how_many, 1, odist, c1, 1, odist, c1, true, effort, p->stream, &fs));
+            TRY(plan_fft_stage(n0, p->dbl, c1, how_many, 1, odist, c1, 1, odist, c1, true, effort, p->stream, &bs));
+            TRY(plan_real_stage(2, n1, p->dbl, (int64_t)n0 * how_many, c1, n1, effort, p->stream, &b));
+            p->stages.push_back(f);
+            p->stages.push_back(fs);
+            p->bwd.push_back(bs);
+            p->bwd.push_back(b);
+        }
+    }
+    *plan = p.release();
+    return DTFFT_HIP_SUCCESS;
+}
+
+int32_t dtfft_hip_fft_execute(void* plan, const void* in, void* out, int8_t sign) {
+    auto* p = (fft_plan*)plan;
+    if (!p) return DTFFT_HIP_ERROR_PLAN_NOT_CREATED;
+    if (sign != DTFFT_HIP_FFT_FORWARD && sign != DTFFT_HIP_FFT_BACKWARD) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    if (p->type == DTFFT_HIP_FFT_R2C) {
+        if (in == out) return DTFFT_HIP_ERROR_INVALID_USAGE;  // src/dtfft_abstract_executor.F90:226-228
+        if (sign == DTFFT_HIP_FFT_FORWARD) {
+            TRY(p->stages[0].run(in, out, sign, p->stream));
+            for (size_t i = 1; i < p->stages.size(); i++) TRY(p->stages[i].run(out, out, sign, p->stream));
+        } else {
+            if (p->bwd.size() == 2) {
+                TRY(p->bwd[0].run(in, (void*)in, sign, p->stream));
+                TRY(p->bwd[1].run(in, out, sign, p->stream));
+            } else {
+                TRY(p->bwd[0].run(in, out, sign, p->stream));
+            }
+        }
+        return DTFFT_HIP_SUCCESS;
+    }
+    TRY(p->stages[0].run(in, out, sign, p->stream));
+    for (size_t i = 1; i < p->stages.size(); i++) TRY(p->stages[i].run(out, out, sign, p->stream));
+    return DTFFT_HIP_SUCCESS;
+}
+
+int32_t dtfft_hip_fft_destroy(void* plan) {
+    delete (fft_plan*)plan;
+    return DTFFT_HIP_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,151 @@
+// Internal declarations shared by the host-side sources of libdtfft_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/dtfft_hip.h"
+
+namespace dtfft_hip {
+
+// Error propagation: functions return int32 codes; HIP_TRY converts a failed
+// HIP call into DTFFT_HIP_ERROR_HIP_RUNTIME after recording the message.
+void set_last_error(const std::string& msg);
+const std::string& last_error();
+
+#define HIP_TRY(call)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            ::dtfft_hip::set_last_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
+            return DTFFT_HIP_ERROR_HIP_RUNTIME;                                            \
+        }                                                                                  \
+    } while (0)
+
+#define TRY(call)                          \
+    do {                                   \
+        int32_t rc_ = (call);              \
+        if (rc_ != DTFFT_HIP_SUCCESS) return rc_; \
+    } while (0)
+
+// ---- device / runtime -------------------------------------------------------------
+struct device_info {
+    bool present = false;      // a gfx950 device is visible and usable
+    int device = -1;
+    std::string arch;          // e.g. "gfx950:sramecc+:xnack-"
+    int num_cus = 0;
+    size_t lds_per_block = 0;
+};
+const device_info& device();
+bool compile_only();           // no launches / allocations (no device or forced)
+
+// ---- hipRTC program cache ----------------------------------------------------------
+// A compiled kernel: module + function.  `source` is one of the embedded device sources
+// ("fft", "transpose"), `expr` a name expression such as "box_copy<8, 2>".
+struct rtc_kernel {
+    hipModule_t module = nullptr;
+    hipFunction_t fn = nullptr;
+    std::string expr;
+};
+// Compiles (or loads from cache) and returns the kernel; nullptr with an error code on failure.
+int32_t rtc_get(const std::string& source, const std::string& expr, const rtc_kernel** out);
+std::string cache_dir();
+
+// Launch helper: kernel args are passed as one packed struct-free pointer array.
+int32_t launch(const rtc_kernel* k, unsigned grid, unsigned block, hipStream_t stream, void** args);
+
+// ---- tuning records (persisted next to the code objects) ---------------------------
+bool tune_lookup(const std::string& key, std::string& value);
+void tune_store(const std::string& key, const std::string& value);
+
+// ---- FFT stage (used by the executor plugin, plans and the batched cycle) ----------
+struct lines_desc {  // mirrors dtfft::lines_desc in device/fft.hip
+    int64_t nl, nb;
+    int64_t in_ls, in_bs, in_ns;
+    int64_t out_ls, out_bs, out_ns;
+    int64_t ntl, ntiles;
+};
+
+// Launch configuration of one batched 1D FFT kernel.
+struct fft_config {
+    int n = 0;
+    std::vector<int> radices;
+    int tpl = 0;   // threads per line
+    int lpb = 0;   // lines per block
+    int pad = 0;   // LDS line padding (complex elements)
+    bool in_cols = false, out_cols = false;
+    bool dbl = false;
+    int kind = 0;  // 0: C2C, 1: R2C (n = 2 * this->n real inputs), 2: C2R
+    std::string expr() const;
+};
+
+// Twiddle table of one length on the device (generated in FP64 on the GPU).
+struct twiddles {
+    void* ptr = nullptr;
+    int64_t bytes = 0;
+    ~twiddles();
+};
+int32_t get_twiddles(int n, const std::vector<int>& radices, bool dbl, std::shared_ptr<twiddles>* out);
+
+bool factorize(int n, std::vector<int>& radices);
+// Chooses tpl / lpb for a length and access mode (heuristic default of the autotuner).
+fft_config default_fft_config(int n, bool dbl, bool in_cols, bool out_cols, bool inplace);
+std::vector<fft_config> fft_candidates(int n, bool dbl, bool in_cols, bool out_cols, bool inplace);
+
+// One batched C2C stage: y = FFT_sign(x) over the lines of `d`.
+struct fft_stage {
+    fft_config cfg;
+    lines_desc d{};
+    const rtc_kernel* k = nullptr;
+    std::shared_ptr<twiddles> tw;
+    std::shared_ptr<twiddles> rtw;  // R2C / C2R untangling twiddles exp(-2 pi i k / 2M)
+    unsigned grid = 0, block = 0;
+    int32_t init(const fft_config& c, const lines_desc& desc);
+    int32_t run(const void* in, void* out, int sign, hipStream_t s) const;
+};
+lines_desc make_lines(int64_t nl, int64_t nb, int64_t in_ls, int64_t in_bs, int64_t in_ns,
+                      int64_t out_ls, int64_t out_bs, int64_t out_ns, int lpb);
+// Picks the configuration for a stage (heuristic, or timed when effort >= MEASURE).
+int32_t plan_fft_stage(int n, bool dbl, int64_t nl, int64_t nb, int64_t in_ls, int64_t in_bs,
+                       int64_t in_ns, int64_t out_ls, int64_t out_bs, int64_t out_ns, bool inplace,
+                       int effort, hipStream_t stream, fft_stage* st);
+// Real-to-complex (kind 1) or complex-to-real (kind 2) stage over contiguous lines of n
+// reals (n even); idist / odist as abstract_executor passes them (reals for the real side).
+int32_t plan_real_stage(int kind, int n, bool dbl, int64_t how_many, int64_t idist, int64_t odist,
+                        int effort, hipStream_t stream, fft_stage* st);
+int32_t get_rtwiddles(int n, bool dbl, std::shared_ptr<twiddles>* out);
+
+// ---- exact data movement (kernel plugin) ----------------------------------------
+struct box_desc {  // mirrors box_desc in device/transpose.hip
+    int64_t e0, e1, e2;
+    int64_t is1, is2;
+    int64_t os0, os1, os2;
+    int64_t din, dout;
+    int64_t t0, t1;
+    int64_t nblocks;
+};
+struct box_move {
+    box_desc d{};
+    bool transpose = false;
+    int es = 0;
+};
+// Launches one box move with the tuned kernel for its element size and shape class.
+struct mover {
+    int es = 0;
+    bool transpose = false;
+    int tile = 0, rows = 0, pad = 0, vec = 1;
+    const rtc_kernel* k = nullptr;
+    const rtc_kernel* k_scalar = nullptr;  // box_copy without vector access (unaligned boxes)
+    int32_t init(int es, bool transpose, int tile, int rows, int pad, int vec);
+    int32_t run(const box_desc& d, const void* in, void* out, hipStream_t s) const;
+};
+int32_t tune_mover(const box_move& m, int effort, hipStream_t stream, mover* out);
+
+}  // namespace dtfft_hip

@@ -1,0 +1,340 @@
+// Kernel plugin: the permute / pack / unpack kernels of a transpose on gfx950.
+//
+// Drop-in for kernel_device (src/dtfft_kernel_device.F90): create follows
+// abstract_kernel%create (src/dtfft_abstract_kernel.F90:219-288) -- zero dims make a
+// dummy kernel, 2D BACKWARD kernels are remapped to FORWARD -- and kernel_device%create
+// (src/dtfft_kernel_device.F90:61-102) -- COPY kernels become device copies.  execute
+// follows kernel_device%execute (src/dtfft_kernel_device.F90:104-178): permutes are one
+// launch, pipelined kernels one launch for the requested neighbour, the non-pipelined
+// PACK / UNPACK / PERMUTE_BACKWARD_END one launch per neighbour.
+//
+// Every kernel type is expressed as a move of a 3D element box (box_desc) whose index
+// maps are those of the host routines in src/include/_dtfft_kernel_host_routines.inc;
+// the box is then executed by either a row copy or an LDS-tiled transpose.  Unlike
+// the CUDA path of the reference, the host-only UNPACK_FORWARD / UNPACK_BACKWARD kinds
+// (routines :757-1021) are supported too.
+#include <algorithm>
+#include <array>
+
+#include "internal.h"
+
+namespace dtfft_hip {
+
+// ---- movers ----------------------------------------------------------------------
+int32_t mover::init(int es_, bool transpose_, int tile_, int rows_, int pad_, int vec_) {
+    es = es_; transpose = transpose_; tile = tile_; rows = rows_; pad = pad_; vec = vec_;
+    std::string expr;
+    if (transpose)
+        expr = "box_transpose<" + std::to_string(es) + ", " + std::to_string(tile) + ", " +
+               std::to_string(rows) + ", " + std::to_string(pad) + ">";
+    else {
+        expr = "box_copy<" + std::to_string(es) + ", " + std::to_string(vec) + ">";
+        TRY(rtc_get("transpose", "box_copy<" + std::to_string(es) + ", 1>", &k_scalar));
+    }
+    return rtc_get("transpose", expr, &k);
+}
+
+static bool aligned_for(const box_desc& d, int vec, const void* in, const void* out, int es);
+
+int32_t mover::run(const box_desc& din, const void* in, void* out, hipStream_t s) const {
+    box_desc d = din;
+    unsigned grid, block;
+    if (d.e0 <= 0 || d.e1 <= 0 || d.e2 <= 0) return DTFFT_HIP_SUCCESS;
+    if (transpose) {
+        d.t0 = (d.e0 + tile - 1) / tile;
+        d.t1 = (d.e1 + tile - 1) / tile;
+        d.nblocks = d.t0 * d.t1 * d.e2;
+        block = (unsigned)(tile * rows);
+        grid = (unsigned)std::min<int64_t>(d.nblocks, 1 << 22);
+    } else {
+        const int v = aligned_for(d, vec, in, out, es) ? vec : 1;
+        const int64_t chunks = (d.e0 + v * 64 - 1) / (v * 64);
+        const int64_t items = chunks * d.e1 * d.e2;  // one wave per item, 4 waves per block
+        d.t0 = d.t1 = 0;
+        d.nblocks = (items + 3) / 4;
+        block = 256;
+        grid = (unsigned)std::min<int64_t>(d.nblocks, 1 << 22);
+        void* args[] = {&out, &in, &d};
+        return launch(v == vec ? k : k_scalar, grid, block, s, args);
+    }
+    void* args[] = {&out, &in, &d};
+    return launch(k, grid, block, s, args);
+}
+
+static bool aligned_for(const box_desc& d, int vec, const void* in, const void* out, int es) {
+    if (vec == 1) return true;
+    auto m = [&](int64_t v) { return v % vec == 0; };
+    return m(d.din) && m(d.dout) && m(d.is1) && m(d.is2) && m(d.os1) && m(d.os2) &&
+           ((uintptr_t)in % (size_t)(vec * es)) == 0 && ((uintptr_t)out % (size_t)(vec * es)) == 0;
+}
+
+// Default launch shapes; the autotuner may replace them (tune_mover).
+static void default_shape(int es, bool transpose, int& tile, int& rows, int& pad, int& vec) {
+    pad = 1;
+    vec = std::max(1, 16 / es);
+    if (transpose) {
+        tile = es == 16 ? 32 : 64;
+        rows = es == 16 ? 8 : 4;
+    } else {
+        tile = rows = 0;
+    }
+}
+
+static double time_mover(const mover& mv, const box_desc& d, void* in, void* out, hipStream_t s,
+                         int warm, int iters) {
+    for (int i = 0; i < warm; i++) if (mv.run(d, in, out, s)) return 1e30;
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess) return 1e30;
+    if (hipEventCreate(&b) != hipSuccess) { hipEventDestroy(a); return 1e30; }
+    hipEventRecord(a, s);
+    for (int i = 0; i < iters; i++) mv.run(d, in, out, s);
+    hipEventRecord(b, s);
+    hipEventSynchronize(b);
+    float ms = 0;
+    hipEventElapsedTime(&ms, a, b);
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    return ms / iters;
+}
+
+int32_t tune_mover(const box_move& m, int effort, hipStream_t stream, mover* out) {
+    int tile, rows, pad, vec;
+    default_shape(m.es, m.transpose, tile, rows, pad, vec);
+    const auto& d = m.d;
+    const std::string key = std::string("mover|") + (m.transpose ? "T" : "C") + "|" + std::to_string(m.es) +
+                            "|" + std::to_string(d.e0) + "x" + std::to_string(d.e1) + "x" + std::to_string(d.e2) +
+                            "|" + std::to_string(d.is1) + "," + std::to_string(d.os0);
+    std::string rec;
+    if (m.transpose && tune_lookup(key, rec)) {
+        int t, r, p;
+        if (std::sscanf(rec.c_str(), "%d %d %d", &t, &r, &p) == 3) { tile = t; rows = r; pad = p; }
+    } else if (m.transpose && effort >= DTFFT_HIP_MEASURE && !compile_only()) {
+        // time candidate shapes on scratch buffers of the box's extent
+        const int64_t span_in = d.din + (d.e0 - 1) + (d.e1 - 1) * d.is1 + (d.e2 - 1) * d.is2 + 1;
+        const int64_t span_out = d.dout + (d.e0 - 1) * d.os0 + (d.e1 - 1) * d.os1 + (d.e2 - 1) * d.os2 + 1;
+        void *bi = nullptr, *bo = nullptr;
+        HIP_TRY(hipMalloc(&bi, (size_t)span_in * m.es));
+        if (hipMalloc(&bo, (size_t)span_out * m.es) != hipSuccess) { hipFree(bi); return DTFFT_HIP_ERROR_ALLOC_FAILED; }
+        double best = 1e30;
+        const int tiles[] = {16, 32, 64};
+        const int rowss[] = {1, 2, 4, 8, 16};
+        const int pads[] = {0, 1, 2};
+        for (int t : tiles)
+            for (int r : rowss)
+                for (int p : pads) {
+                    if (t * r > 1024 || t * r < 64 || r > t) continue;
+                    if ((size_t)t * (t + p) * m.es > 64 * 1024) continue;
+                    if (effort < DTFFT_HIP_EXHAUSTIVE && p == 2) continue;
+                    mover cand;
+                    if (cand.init(m.es, true, t, r, p, 1)) continue;
+                    double ms = time_mover(cand, d, bi, bo, stream, 2, 5);
+                    if (ms < best) { best = ms; tile = t; rows = r; pad = p; }
+                }
+        hipStreamSynchronize(stream);
+        hipFree(bi);
+        hipFree(bo);
+        char buf[64];
+        std::snprintf(buf, sizeof buf, "%d %d %d", tile, rows, pad);
+        tune_store(key, buf);
+    }
+    return out->init(m.es, m.transpose, tile, rows, pad, m.transpose ? 1 : vec);
+}
+
+// ---- kernel type -> box -------------------------------------------------------------
+// Index maps of _dtfft_kernel_host_routines.inc (0-based).  nd = (nxx,nyy,nzz,din,dout).
+static int32_t make_box(int kt, int ndims, const int* dims, const int* nd, box_move& m) {
+    const int64_t nx = dims[0], ny = dims[1], nz = ndims == 3 ? dims[2] : 1;
+    box_desc& d = m.d;
+    d = box_desc{};
+    const int64_t nxx = nd ? nd[0] : 0, nyy = nd ? nd[1] : 0, nzz = nd ? (ndims == 3 ? nd[2] : 1) : 0;
+    const int64_t din = nd ? nd[3] : 0, dout = nd ? nd[4] : 0;
+    switch (kt) {
+        case DTFFT_HIP_KERNEL_PERMUTE_FORWARD:  // :140-196
+            m.transpose = true;
+            if (ndims == 2) { d.e0 = nx; d.e1 = ny; d.e2 = 1; d.is1 = nx; d.os0 = ny; d.os1 = 1; }
+            else { d.e0 = nx; d.e1 = ny; d.e2 = nz; d.is1 = nx; d.is2 = nx * ny; d.os0 = ny * nz; d.os1 = 1; d.os2 = ny; }
+            return 0;
+        case DTFFT_HIP_KERNEL_PERMUTE_BACKWARD:  // :262-302, out(z + x*nz + y*nz*nx)
+            m.transpose = true;
+            d.e0 = nx; d.e1 = nz; d.e2 = ny; d.is1 = nx * ny; d.is2 = nx; d.os0 = nz; d.os1 = 1; d.os2 = nz * nx;
+            return 0;
+        case DTFFT_HIP_KERNEL_PERMUTE_BACKWARD_START:  // :351-390, out(z + y*nz + x*nz*ny)
+            m.transpose = true;
+            d.e0 = nx; d.e1 = nz; d.e2 = ny; d.is1 = nx * ny; d.is2 = nx; d.os0 = nz * ny; d.os1 = 1; d.os2 = nz;
+            return 0;
+        case DTFFT_HIP_KERNEL_PERMUTE_BACKWARD_END:
+        case DTFFT_HIP_KERNEL_PERMUTE_BACKWARD_END_PIPELINED:  // :439-482
+            m.transpose = false;
+            d.e0 = nxx; d.e1 = nyy; d.e2 = nzz; d.is1 = nzz * nxx; d.is2 = nxx; d.os0 = 1; d.os1 = nx; d.os2 = nx * ny;
+            d.din = din; d.dout = dout;
+            return 0;
+        case DTFFT_HIP_KERNEL_UNPACK:
+        case DTFFT_HIP_KERNEL_UNPACK_PIPELINED:  // :575-636
+            m.transpose = false;
+            d.e0 = nxx; d.e1 = nyy; d.e2 = nzz; d.is1 = nxx; d.is2 = nxx * nyy; d.os0 = 1; d.os1 = nx; d.os2 = nx * ny;
+            d.din = din; d.dout = dout;
+            return 0;
+        case DTFFT_HIP_KERNEL_PACK:
+        case DTFFT_HIP_KERNEL_PACK_PIPELINED:  // :657-721
+            m.transpose = false;
+            d.e0 = nxx; d.e1 = nyy; d.e2 = nzz; d.is1 = nx; d.is2 = nx * ny; d.os0 = 1; d.os1 = nxx; d.os2 = nxx * nyy;
+            d.din = din; d.dout = dout;
+            return 0;
+        case DTFFT_HIP_KERNEL_PACK_FORWARD:  // :1026-1086
+            m.transpose = true;
+            if (ndims == 2) { d.e0 = nxx; d.e1 = nyy; d.e2 = 1; d.is1 = nx; d.os0 = nyy; d.os1 = 1; }
+            else { d.e0 = nxx; d.e1 = nyy; d.e2 = nzz; d.is1 = nx; d.is2 = nx * ny; d.os0 = nyy * nzz; d.os1 = 1; d.os2 = nyy; }
+            d.din = din; d.dout = dout;
+            return 0;
+        case DTFFT_HIP_KERNEL_PACK_BACKWARD:  // :1155-1198, out(z + x*nzz + y*nzz*nxx)
+            m.transpose = true;
+            d.e0 = nxx; d.e1 = nzz; d.e2 = nyy; d.is1 = nx * ny; d.is2 = nx; d.os0 = nzz; d.os1 = 1; d.os2 = nzz * nxx;
+            d.din = din; d.dout = dout;
+            return 0;
+        case DTFFT_HIP_KERNEL_UNPACK_FORWARD:
+        case DTFFT_HIP_KERNEL_UNPACK_FORWARD_PIPELINED:  // :757-811
+            m.transpose = true;
+            if (ndims == 2) {  // in(x*nyy + y) -> out(y*nx + x): contiguous input axis is y
+                d.e0 = nyy; d.e1 = nxx; d.e2 = 1; d.is1 = nyy; d.os0 = nx; d.os1 = 1;
+            } else {  // in(z + x*nzz + y*nzz*nxx) -> out(x + y*nx + z*nx*ny)
+                d.e0 = nzz; d.e1 = nxx; d.e2 = nyy; d.is1 = nzz; d.is2 = nzz * nxx; d.os0 = nx * ny; d.os1 = 1; d.os2 = nx;
+            }
+            d.din = din; d.dout = dout;
+            return 0;
+        case DTFFT_HIP_KERNEL_UNPACK_BACKWARD:
+        case DTFFT_HIP_KERNEL_UNPACK_BACKWARD_PIPELINED:  // :907-946, in(y + z*nyy + x*nzz*nyy)
+            m.transpose = true;
+            d.e0 = nyy; d.e1 = nxx; d.e2 = nzz; d.is1 = nzz * nyy; d.is2 = nyy; d.os0 = nx; d.os1 = 1; d.os2 = nx * ny;
+            d.din = din; d.dout = dout;
+            return 0;
+        default:
+            return DTFFT_HIP_ERROR_INVALID_KERNEL_TYPE;
+    }
+}
+
+static bool is_pipelined_only(int kt) {
+    return kt == DTFFT_HIP_KERNEL_UNPACK_PIPELINED || kt == DTFFT_HIP_KERNEL_PERMUTE_BACKWARD_END_PIPELINED ||
+           kt == DTFFT_HIP_KERNEL_COPY_PIPELINED || kt == DTFFT_HIP_KERNEL_PACK_BACKWARD ||
+           kt == DTFFT_HIP_KERNEL_PACK_FORWARD || kt == DTFFT_HIP_KERNEL_PACK_PIPELINED ||
+           kt == DTFFT_HIP_KERNEL_UNPACK_FORWARD_PIPELINED || kt == DTFFT_HIP_KERNEL_UNPACK_BACKWARD_PIPELINED;
+}
+static bool needs_neighbors(int kt) {
+    return !(kt == DTFFT_HIP_KERNEL_PERMUTE_FORWARD || kt == DTFFT_HIP_KERNEL_PERMUTE_BACKWARD ||
+             kt == DTFFT_HIP_KERNEL_PERMUTE_BACKWARD_START || kt == DTFFT_HIP_KERNEL_COPY);
+}
+
+struct kernel_handle {
+    int kernel_type = 0;
+    int ndims = 0;
+    int dims[3] = {1, 1, 1};
+    int es = 0;
+    bool dummy = false;
+    std::vector<std::array<int, 5>> nd;
+    std::vector<box_move> boxes;   // one per neighbour (or one for permutes)
+    std::vector<mover> movers;     // matching boxes
+};
+
+}  // namespace dtfft_hip
+
+using namespace dtfft_hip;
+
+extern "C" {
+
+int32_t dtfft_hip_kernel_create(int32_t kernel_type, int32_t ndims, const int32_t* dims, int64_t base_storage,
+                                const int32_t* neighbor_data, int32_t n_peers, int32_t effort,
+                                int32_t force_effort, void** handle) {
+    (void)force_effort;
+    if (!handle || !dims) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    *handle = nullptr;
+    if (ndims != 2 && ndims != 3) return DTFFT_HIP_ERROR_INVALID_N_DIMENSIONS;
+    if (base_storage != 4 && base_storage != 8 && base_storage != 16) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    if (effort < DTFFT_HIP_ESTIMATE || effort > DTFFT_HIP_EXHAUSTIVE) return DTFFT_HIP_ERROR_INVALID_EFFORT_FLAG;
+    auto h = std::make_unique<kernel_handle>();
+    h->ndims = ndims;
+    h->es = (int)base_storage;
+    int64_t total = 1;
+    for (int i = 0; i < ndims; i++) {
+        if (dims[i] < 0) return DTFFT_HIP_ERROR_INVALID_DIMENSION_SIZE;
+        h->dims[i] = dims[i];
+        total *= dims[i];
+        if (dims[i] == 0) h->dummy = true;  // src/dtfft_abstract_kernel.F90:233-237
+    }
+    if (total > INT32_MAX) return DTFFT_HIP_ERROR_INDEX_OVERFLOW;  // check_if_overflow
+    // 2D remap, src/dtfft_abstract_kernel.F90:272-283
+    int kt = kernel_type;
+    if (ndims == 2) {
+        if (kt == DTFFT_HIP_KERNEL_PACK_BACKWARD) kt = DTFFT_HIP_KERNEL_PACK_FORWARD;
+        else if (kt == DTFFT_HIP_KERNEL_PERMUTE_BACKWARD) kt = DTFFT_HIP_KERNEL_PERMUTE_FORWARD;
+        else if (kt == DTFFT_HIP_KERNEL_UNPACK_BACKWARD) kt = DTFFT_HIP_KERNEL_UNPACK_FORWARD;
+        else if (kt == DTFFT_HIP_KERNEL_UNPACK_BACKWARD_PIPELINED) kt = DTFFT_HIP_KERNEL_UNPACK_FORWARD_PIPELINED;
+    }
+    if (ndims == 2 && (kt == DTFFT_HIP_KERNEL_PERMUTE_BACKWARD_START || kt == DTFFT_HIP_KERNEL_PERMUTE_BACKWARD_END ||
+                       kt == DTFFT_HIP_KERNEL_PERMUTE_BACKWARD_END_PIPELINED))
+        return DTFFT_HIP_ERROR_INVALID_KERNEL_TYPE;  // 2-step permute is 3D only
+    h->kernel_type = kt;
+    if (kt == DTFFT_HIP_KERNEL_DUMMY) h->dummy = true;
+    if (h->dummy) { *handle = h.release(); return DTFFT_HIP_SUCCESS; }
+    if (needs_neighbors(kt)) {
+        if (!neighbor_data || n_peers <= 0) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+        for (int p = 0; p < n_peers; p++) {
+            std::array<int, 5> a;
+            for (int r = 0; r < 5; r++) a[r] = neighbor_data[p * 5 + r];
+            h->nd.push_back(a);
+        }
+    }
+    if (kt == DTFFT_HIP_KERNEL_COPY || kt == DTFFT_HIP_KERNEL_COPY_PIPELINED) {
+        *handle = h.release();
+        return DTFFT_HIP_SUCCESS;  // device copies, src/dtfft_kernel_device.F90:77-81
+    }
+    hipStream_t s = nullptr;
+    const size_t nb = needs_neighbors(kt) ? h->nd.size() : 1;
+    for (size_t p = 0; p < nb; p++) {
+        box_move m;
+        m.es = h->es;
+        int32_t rc = make_box(kt, ndims, h->dims, needs_neighbors(kt) ? h->nd[p].data() : nullptr, m);
+        if (rc) return rc;
+        mover mv;
+        rc = tune_mover(m, effort, s, &mv);
+        if (rc) return rc;
+        h->boxes.push_back(m);
+        h->movers.push_back(mv);
+    }
+    *handle = h.release();
+    return DTFFT_HIP_SUCCESS;
+}
+
+int32_t dtfft_hip_kernel_execute(void* handle, const void* in, void* out, void* stream, int32_t neighbor,
+                                 int32_t sync) {
+    auto* h = (kernel_handle*)handle;
+    if (!h) return DTFFT_HIP_ERROR_PLAN_NOT_CREATED;
+    if (h->dummy) return DTFFT_HIP_SUCCESS;
+    hipStream_t s = (hipStream_t)stream;
+    const int kt = h->kernel_type;
+    const int np = (int)h->nd.size();
+    if (is_pipelined_only(kt) && (neighbor < 1 || neighbor > np)) return DTFFT_HIP_ERROR_INVALID_NEIGHBOR;
+    if (neighbor < 0 || neighbor > np) return DTFFT_HIP_ERROR_INVALID_NEIGHBOR;
+    if (kt == DTFFT_HIP_KERNEL_COPY) {
+        size_t bytes = (size_t)h->es * (size_t)h->dims[0] * h->dims[1] * h->dims[2];
+        HIP_TRY(hipMemcpyAsync(out, in, bytes, hipMemcpyDeviceToDevice, s));
+    } else if (kt == DTFFT_HIP_KERNEL_COPY_PIPELINED) {
+        auto& a = h->nd[neighbor - 1];
+        size_t bytes = (size_t)h->es * (size_t)a[0] * a[1] * a[2];
+        HIP_TRY(hipMemcpyAsync((char*)out + (size_t)h->es * a[4], (const char*)in + (size_t)h->es * a[3], bytes,
+                               hipMemcpyDeviceToDevice, s));
+    } else if (!needs_neighbors(kt)) {
+        TRY(h->movers[0].run(h->boxes[0].d, in, out, s));
+    } else if (neighbor >= 1) {
+        TRY(h->movers[neighbor - 1].run(h->boxes[neighbor - 1].d, in, out, s));
+    } else {
+        for (int p = 0; p < np; p++) TRY(h->movers[p].run(h->boxes[p].d, in, out, s));
+    }
+    if (sync) HIP_TRY(hipStreamSynchronize(s));
+    return DTFFT_HIP_SUCCESS;
+}
+
+int32_t dtfft_hip_kernel_destroy(void* handle) {
+    delete (kernel_handle*)handle;
+    return DTFFT_HIP_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,489 @@
+// Single-rank plans: the dtFFT plan API (include/dtfft.h) for one process and one GPU.
+//
+// A plan owns the FFT stages and the transposes of the pencil cycle.  The sequences
+// follow src/dtfft_plan.F90:
+//   execute_2d      (:877-913)   FFT_X, X->Y, FFT_Y
+//   execute_z_slab  (:974-1016)  2D FFT over X,Y, X->Z, FFT_Z
+//   execute_generic (:1057-1101) FFT_X, X->Y, FFT_Y, Y->Z, FFT_Z
+// with the reference's buffer ping-pong between in / aux / out.  On one rank every
+// transpose is a single permute kernel (src/dtfft_reshape_handle_generic.F90:232-253):
+// X->Y, Y->Z, Z->X are PERMUTE_FORWARD, the reverse directions PERMUTE_BACKWARD.
+//
+// With enable_fused (default for C2C plans) each transpose is fused into the FFT pass
+// that follows it: the kernel reads the pencil with the FFT axis strided and writes the
+// next pencil with the FFT axis contiguous, so a 3D direction is three HBM passes
+// (in -> out -> aux -> out) instead of five.  R2C plans run the plain sequence.
+#include <algorithm>
+#include <cstring>
+
+#include "internal.h"
+
+extern "C" int32_t dtfft_hip_kernel_create(int32_t, int32_t, const int32_t*, int64_t, const int32_t*, int32_t,
+                                           int32_t, int32_t, void**);
+extern "C" int32_t dtfft_hip_kernel_execute(void*, const void*, void*, void*, int32_t, int32_t);
+extern "C" int32_t dtfft_hip_kernel_destroy(void*);
+
+namespace dtfft_hip {
+
+static dtfft_hip_config_t g_config = {0, 1, 0, 2, 5, nullptr};
+
+struct permute {
+    void* h = nullptr;
+    permute() = default;
+    permute(const permute&) = delete;
+    permute& operator=(const permute&) = delete;
+    ~permute() { if (h) dtfft_hip_kernel_destroy(h); }
+    int32_t create(int kt, int ndims, int64_t a, int64_t b, int64_t c, int es, int effort) {
+        int32_t d[3] = {(int32_t)a, (int32_t)b, (int32_t)c};
+        return dtfft_hip_kernel_create(kt, ndims, d, es, nullptr, 0, effort, 0, &h);
+    }
+    int32_t run(const void* in, void* out, hipStream_t s) const {
+        if (!h) return DTFFT_HIP_ERROR_PLAN_NOT_CREATED;
+        return dtfft_hip_kernel_execute(h, in, out, (void*)s, 0, 0);
+    }
+};
+
+struct plan_t {
+    int ndims = 0;
+    int64_t nx = 1, ny = 1, nz = 1;  // global dims, X fastest
+    int64_t nc = 1;                  // complex extent of X (nx/2+1 for R2C)
+    bool r2c = false, dbl = false, fused = false, z_slab = false;
+    bool transpose_only = false;
+    int effort = 0;
+    hipStream_t stream = nullptr;
+    size_t elem_size = 8;            // dtfft_get_element_size: complex bytes, reals for R2C
+    size_t cplx = 8;                 // bytes of one complex element
+    size_t alloc_elems = 0;
+    void* aux = nullptr;             // allocated lazily, src/dtfft_plan.F90:2286-2331
+
+    permute t_xy, t_yx, t_yz, t_zy, t_xz, t_zx;
+    fft_stage fx, fx_c2r, fy, fz, fy_cols;           // plain stages
+    fft_stage u_xy, u_yz, u_zy, u_yx, u_xz, u_zx;    // fused stages
+
+    plan_t() = default;
+    plan_t(const plan_t&) = delete;
+    ~plan_t() { if (aux) hipFree(aux); }
+};
+
+static int32_t ensure_aux(plan_t* p, void*& aux) {
+    if (aux) return DTFFT_HIP_SUCCESS;
+    if (!p->aux) {
+        if (compile_only()) return DTFFT_HIP_ERROR_NO_DEVICE;
+        if (hipMalloc(&p->aux, std::max<size_t>(1, p->alloc_elems * p->elem_size)) != hipSuccess) {
+            p->aux = nullptr;
+            return DTFFT_HIP_ERROR_ALLOC_FAILED;
+        }
+    }
+    aux = p->aux;
+    return DTFFT_HIP_SUCCESS;
+}
+
+// Contiguous C2C stage over `lines` lines of length n.
+static int32_t contig(plan_t* p, int64_t n, int64_t lines, fft_stage* st) {
+    return plan_fft_stage((int)n, p->dbl, lines, 1, n, 0, 1, n, 0, 1, false, p->effort, p->stream, st);
+}
+
+// Fused transpose + FFT: lines (l, b) with l contiguous in the input, FFT axis stride ns;
+// output written with the FFT axis contiguous.
+static int32_t fused(plan_t* p, int64_t n, int64_t nl, int64_t nb, int64_t in_bs, int64_t in_ns, int64_t out_ls,
+                     int64_t out_bs, fft_stage* st) {
+    return plan_fft_stage((int)n, p->dbl, nl, nb, 1, in_bs, in_ns, out_ls, out_bs, 1, false, p->effort, p->stream,
+                          st);
+}
+
+static int32_t build(plan_t* p) {
+    const int es = (int)p->cplx;
+    const int64_t nx = p->nx, ny = p->ny, nz = p->nz, nc = p->nc;
+    const int E = p->effort;
+    if (p->ndims == 2) {
+        TRY(p->t_xy.create(DTFFT_HIP_KERNEL_PERMUTE_FORWARD, 2, nc, ny, 1, es, E));
+        TRY(p->t_yx.create(DTFFT_HIP_KERNEL_PERMUTE_BACKWARD, 2, ny, nc, 1, es, E));
+        if (p->transpose_only) return DTFFT_HIP_SUCCESS;
+        if (p->r2c) {
+            TRY(plan_real_stage(1, (int)nx, p->dbl, ny, nx, nc, E, p->stream, &p->fx));
+            TRY(plan_real_stage(2, (int)nx, p->dbl, ny, nc, nx, E, p->stream, &p->fx_c2r));
+        } else {
+            TRY(contig(p, nx, ny, &p->fx));
+        }
+        TRY(contig(p, ny, nc, &p->fy));
+        if (p->fused) {
+            TRY(fused(p, ny, nc, 1, 0, nc, ny, 0, &p->u_xy));   // X (nc,ny) -> Y (ny,nc)
+            TRY(fused(p, nx, ny, 1, 0, ny, nx, 0, &p->u_yx));   // Y (ny,nx) -> X (nx,ny)
+        }
+        return DTFFT_HIP_SUCCESS;
+    }
+    // 3D pencils: X = (nc, ny, nz), Y = (ny, nz, nc), Z = (nz, nc, ny)
+    if (p->z_slab) {
+        TRY(p->t_xz.create(DTFFT_HIP_KERNEL_PERMUTE_BACKWARD, 3, nc, ny, nz, es, E));
+        TRY(p->t_zx.create(DTFFT_HIP_KERNEL_PERMUTE_FORWARD, 3, nz, nc, ny, es, E));
+    } else {
+        TRY(p->t_xy.create(DTFFT_HIP_KERNEL_PERMUTE_FORWARD, 3, nc, ny, nz, es, E));
+        TRY(p->t_yz.create(DTFFT_HIP_KERNEL_PERMUTE_FORWARD, 3, ny, nz, nc, es, E));
+        TRY(p->t_zy.create(DTFFT_HIP_KERNEL_PERMUTE_BACKWARD, 3, nz, nc, ny, es, E));
+        TRY(p->t_yx.create(DTFFT_HIP_KERNEL_PERMUTE_BACKWARD, 3, ny, nz, nc, es, E));
+    }
+    if (p->transpose_only) return DTFFT_HIP_SUCCESS;
+    if (p->r2c) {
+        TRY(plan_real_stage(1, (int)nx, p->dbl, ny * nz, nx, nc, E, p->stream, &p->fx));
+        TRY(plan_real_stage(2, (int)nx, p->dbl, ny * nz, nc, nx, E, p->stream, &p->fx_c2r));
+    } else {
+        TRY(contig(p, nx, ny * nz, &p->fx));
+    }
+    TRY(contig(p, nz, nc * ny, &p->fz));
+    if (p->z_slab) {
+        // Y along the strided axis of the X-pencil (second half of the 2D FFT), in place
+        TRY(plan_fft_stage((int)ny, p->dbl, nc, nz, 1, nc * ny, nc, 1, nc * ny, nc, true, E, p->stream, &p->fy_cols));
+        if (p->fused) {
+            TRY(fused(p, nz, nc, ny, nc, nc * ny, nz, nz * nc, &p->u_xz));  // X -> Z + FFT_Z
+            TRY(fused(p, nx, nz, ny, nz * nc, nz, nc * ny, nc, &p->u_zx));  // Z -> X + FFT_X
+        }
+        return DTFFT_HIP_SUCCESS;
+    }
+    TRY(contig(p, ny, nz * nc, &p->fy));
+    if (p->fused) {
+        TRY(fused(p, ny, nc, nz, nc * ny, nc, ny * nz, ny, &p->u_xy));      // X -> Y + FFT_Y
+        TRY(fused(p, nz, ny, nc, ny * nz, ny, nz * nc, nz, &p->u_yz));      // Y -> Z + FFT_Z
+        TRY(fused(p, ny, nz, nc, nz, nz * nc, ny, ny * nz, &p->u_zy));      // Z -> Y + FFT_Y
+        TRY(fused(p, nx, ny, nz, ny, ny * nz, nc, nc * ny, &p->u_yx));      // Y -> X + FFT_X
+    }
+    return DTFFT_HIP_SUCCESS;
+}
+
+static int32_t create_plan(int8_t ndims, const int32_t* dims, int32_t precision, int32_t effort, int32_t executor,
+                           bool r2c, dtfft_hip_plan_t* out) {
+    if (!out) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (ndims != 2 && ndims != 3) return DTFFT_HIP_ERROR_INVALID_N_DIMENSIONS;
+    if (!dims) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    for (int i = 0; i < ndims; i++) if (dims[i] <= 0) return DTFFT_HIP_ERROR_INVALID_DIMENSION_SIZE;
+    if (precision != DTFFT_HIP_SINGLE && precision != DTFFT_HIP_DOUBLE) return DTFFT_HIP_ERROR_INVALID_PRECISION;
+    if (effort < DTFFT_HIP_ESTIMATE || effort > DTFFT_HIP_EXHAUSTIVE) return DTFFT_HIP_ERROR_INVALID_EFFORT_FLAG;
+    if (executor != DTFFT_HIP_EXECUTOR_NONE && executor != DTFFT_HIP_EXECUTOR_HIP)
+        return DTFFT_HIP_ERROR_INVALID_EXECUTOR_TYPE;
+    if (r2c && executor == DTFFT_HIP_EXECUTOR_NONE) return DTFFT_HIP_ERROR_R2C_TRANSPOSE_PLAN;
+    auto p = std::make_unique<plan_t>();
+    p->ndims = ndims;
+    // dims are given in reverse order (slowest first), include/dtfft.h:436
+    p->nx = dims[ndims - 1];
+    p->ny = dims[ndims - 2];
+    p->nz = ndims == 3 ? dims[0] : 1;
+    p->r2c = r2c;
+    p->dbl = precision == DTFFT_HIP_DOUBLE;
+    p->transpose_only = executor == DTFFT_HIP_EXECUTOR_NONE;
+    p->effort = g_config.enable_kernel_autotune && effort == DTFFT_HIP_ESTIMATE ? DTFFT_HIP_MEASURE : effort;
+    p->stream = (hipStream_t)g_config.stream;
+    p->z_slab = ndims == 3 && g_config.enable_z_slab != 0;
+    p->fused = g_config.enable_fused != 0 && !r2c && !p->transpose_only;
+    p->nc = r2c ? p->nx / 2 + 1 : p->nx;
+    p->cplx = p->dbl ? 16 : 8;
+    p->elem_size = r2c ? p->cplx / 2 : p->cplx;
+    const int64_t ncplx = p->nc * p->ny * p->nz;
+    const int64_t nreal = p->nx * p->ny * p->nz;
+    if (ncplx > INT32_MAX || nreal > INT32_MAX) return DTFFT_HIP_ERROR_INDEX_OVERFLOW;
+    p->alloc_elems = r2c ? (size_t)std::max(nreal, 2 * ncplx) : (size_t)ncplx;
+    TRY(build(p.get()));
+    *out = p.release();
+    return DTFFT_HIP_SUCCESS;
+}
+
+// ---- execution -------------------------------------------------------------------------
+// Step runner: executes every step of a sequence, or only step `only` (profiling entry
+// point dtfft_hip_plan_execute_stage); counts the steps either way.
+struct steps {
+    int only = -1;
+    int count = 0;
+};
+#define STEP(r, expr)                                       \
+    do {                                                    \
+        if ((r).only < 0 || (r).only == (r).count) TRY(expr); \
+        (r).count++;                                        \
+    } while (0)
+
+static int32_t exec_transpose_only(plan_t* p, void* in, void* out, bool fwd, void* aux, steps& r) {
+    hipStream_t s = p->stream;
+    if (p->ndims == 2) { STEP(r, fwd ? p->t_xy.run(in, out, s) : p->t_yx.run(in, out, s)); return 0; }
+    if (p->z_slab) { STEP(r, fwd ? p->t_xz.run(in, out, s) : p->t_zx.run(in, out, s)); return 0; }
+    TRY(ensure_aux(p, aux));
+    if (fwd) {
+        STEP(r, p->t_xy.run(in, aux, s));
+        STEP(r, p->t_yz.run(aux, out, s));
+    } else {
+        STEP(r, p->t_zy.run(in, aux, s));
+        STEP(r, p->t_yx.run(aux, out, s));
+    }
+    return DTFFT_HIP_SUCCESS;
+}
+
+static int32_t exec_plain(plan_t* p, void* in, void* out, bool fwd, void* aux, steps& r) {
+    hipStream_t s = p->stream;
+    const int F = DTFFT_HIP_FFT_FORWARD, B = DTFFT_HIP_FFT_BACKWARD;
+    TRY(ensure_aux(p, aux));
+    const fft_stage& fxb = p->r2c ? p->fx_c2r : p->fx;
+    if (p->ndims == 2) {  // execute_2d, src/dtfft_plan.F90:904-912
+        if (fwd) {
+            STEP(r, p->fx.run(in, aux, F, s));
+            STEP(r, p->t_xy.run(aux, out, s));
+            STEP(r, p->fy.run(out, out, F, s));
+        } else {
+            STEP(r, p->fy.run(in, in, B, s));
+            STEP(r, p->t_yx.run(in, aux, s));
+            STEP(r, fxb.run(aux, out, B, s));
+        }
+        return DTFFT_HIP_SUCCESS;
+    }
+    if (p->z_slab) {  // execute_z_slab, src/dtfft_plan.F90:1007-1015
+        if (fwd) {
+            STEP(r, p->fx.run(in, aux, F, s));
+            STEP(r, p->fy_cols.run(aux, aux, F, s));
+            STEP(r, p->t_xz.run(aux, out, s));
+            STEP(r, p->fz.run(out, out, F, s));
+        } else {
+            STEP(r, p->fz.run(in, in, B, s));
+            STEP(r, p->t_zx.run(in, aux, s));
+            STEP(r, p->fy_cols.run(aux, aux, B, s));
+            STEP(r, fxb.run(aux, out, B, s));
+        }
+        return DTFFT_HIP_SUCCESS;
+    }
+    if (fwd) {  // execute_generic, src/dtfft_plan.F90:1088-1100
+        STEP(r, p->fx.run(in, aux, F, s));
+        STEP(r, p->t_xy.run(aux, out, s));
+        STEP(r, p->fy.run(out, out, F, s));
+        STEP(r, p->t_yz.run(out, aux, s));
+        STEP(r, p->fz.run(aux, out, F, s));
+    } else {
+        STEP(r, p->fz.run(in, aux, B, s));
+        STEP(r, p->t_zy.run(aux, in, s));
+        STEP(r, p->fy.run(in, in, B, s));
+        STEP(r, p->t_yx.run(in, aux, s));
+        STEP(r, fxb.run(aux, out, B, s));
+    }
+    return DTFFT_HIP_SUCCESS;
+}
+
+static int32_t exec_fused(plan_t* p, void* in, void* out, bool fwd, void* aux, steps& r) {
+    hipStream_t s = p->stream;
+    const int F = DTFFT_HIP_FFT_FORWARD, B = DTFFT_HIP_FFT_BACKWARD;
+    TRY(ensure_aux(p, aux));
+    if (p->ndims == 2) {
+        if (fwd) {
+            STEP(r, p->fx.run(in, aux, F, s));
+            STEP(r, p->u_xy.run(aux, out, F, s));
+        } else {
+            STEP(r, p->fy.run(in, aux, B, s));
+            STEP(r, p->u_yx.run(aux, out, B, s));
+        }
+        return DTFFT_HIP_SUCCESS;
+    }
+    if (p->z_slab) {
+        if (fwd) {
+            STEP(r, p->fx.run(in, aux, F, s));
+            STEP(r, p->fy_cols.run(aux, aux, F, s));
+            STEP(r, p->u_xz.run(aux, out, F, s));
+        } else {
+            STEP(r, p->fz.run(in, aux, B, s));
+            STEP(r, p->u_zx.run(aux, out, B, s));
+            STEP(r, p->fy_cols.run(out, out, B, s));
+        }
+        return DTFFT_HIP_SUCCESS;
+    }
+    if (fwd) {
+        STEP(r, p->fx.run(in, out, F, s));
+        STEP(r, p->u_xy.run(out, aux, F, s));
+        STEP(r, p->u_yz.run(aux, out, F, s));
+    } else {
+        STEP(r, p->fz.run(in, out, B, s));
+        STEP(r, p->u_zy.run(out, aux, B, s));
+        STEP(r, p->u_yx.run(aux, out, B, s));
+    }
+    return DTFFT_HIP_SUCCESS;
+}
+
+static int32_t exec_steps(plan_t* p, void* in, void* out, bool fwd, void* aux, steps& r) {
+    if (p->transpose_only) return exec_transpose_only(p, in, out, fwd, aux, r);
+    return p->fused ? exec_fused(p, in, out, fwd, aux, r) : exec_plain(p, in, out, fwd, aux, r);
+}
+
+}  // namespace dtfft_hip
+
+using namespace dtfft_hip;
+
+extern "C" {
+
+int32_t dtfft_hip_create_config(dtfft_hip_config_t* c) {
+    if (!c) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    *c = dtfft_hip_config_t{0, 1, 0, 2, 5, nullptr};
+    return DTFFT_HIP_SUCCESS;
+}
+
+int32_t dtfft_hip_set_config(const dtfft_hip_config_t* c) {
+    if (!c) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    if (c->measure_warmup_iters < 0) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    if (c->measure_iters <= 0) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    g_config = *c;
+    return DTFFT_HIP_SUCCESS;
+}
+
+int32_t dtfft_hip_create_plan_c2c(int8_t ndims, const int32_t* dims, int32_t precision, int32_t effort,
+                                  int32_t executor, dtfft_hip_plan_t* plan) {
+    return create_plan(ndims, dims, precision, effort, executor, false, plan);
+}
+
+int32_t dtfft_hip_create_plan_r2c(int8_t ndims, const int32_t* dims, int32_t precision, int32_t effort,
+                                  int32_t executor, dtfft_hip_plan_t* plan) {
+    return create_plan(ndims, dims, precision, effort, executor, true, plan);
+}
+
+int32_t dtfft_hip_execute(dtfft_hip_plan_t plan, void* in, void* out, int32_t execute_type, void* aux) {
+    auto* p = (plan_t*)plan;
+    if (!p) return DTFFT_HIP_ERROR_PLAN_NOT_CREATED;
+    if (execute_type != DTFFT_HIP_EXECUTE_FORWARD && execute_type != DTFFT_HIP_EXECUTE_BACKWARD)
+        return DTFFT_HIP_ERROR_INVALID_EXECUTE_TYPE;
+    if (!in || !out) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    if (aux && (aux == in || aux == out)) return DTFFT_HIP_ERROR_INVALID_AUX;
+    if (compile_only()) return DTFFT_HIP_ERROR_NO_DEVICE;
+    const bool fwd = execute_type == DTFFT_HIP_EXECUTE_FORWARD;
+    if (p->transpose_only && in == out) return DTFFT_HIP_ERROR_INPLACE_TRANSPOSE;
+    steps r;
+    return exec_steps(p, in, out, fwd, aux, r);
+}
+
+int32_t dtfft_hip_plan_execute_stage(dtfft_hip_plan_t plan, void* in, void* out, int32_t execute_type, void* aux,
+                                     int32_t stage, int32_t* n_stages) {
+    auto* p = (plan_t*)plan;
+    if (!p) return DTFFT_HIP_ERROR_PLAN_NOT_CREATED;
+    if (execute_type != DTFFT_HIP_EXECUTE_FORWARD && execute_type != DTFFT_HIP_EXECUTE_BACKWARD)
+        return DTFFT_HIP_ERROR_INVALID_EXECUTE_TYPE;
+    if (compile_only()) return DTFFT_HIP_ERROR_NO_DEVICE;
+    steps r;
+    r.only = stage < 0 ? INT32_MAX : stage;  // stage < 0: count only
+    TRY(exec_steps(p, in, out, execute_type == DTFFT_HIP_EXECUTE_FORWARD, aux, r));
+    if (n_stages) *n_stages = r.count;
+    if (stage >= r.count) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    return DTFFT_HIP_SUCCESS;
+}
+
+int32_t dtfft_hip_transpose(dtfft_hip_plan_t plan, void* in, void* out, int32_t tt, void* aux) {
+    (void)aux;
+    auto* p = (plan_t*)plan;
+    if (!p) return DTFFT_HIP_ERROR_PLAN_NOT_CREATED;
+    if (in == out) return DTFFT_HIP_ERROR_INPLACE_TRANSPOSE;
+    if (compile_only()) return DTFFT_HIP_ERROR_NO_DEVICE;
+    hipStream_t s = p->stream;
+    if (p->ndims == 2) {
+        if (tt == DTFFT_HIP_TRANSPOSE_X_TO_Y) return p->t_xy.run(in, out, s);
+        if (tt == DTFFT_HIP_TRANSPOSE_Y_TO_X) return p->t_yx.run(in, out, s);
+        return DTFFT_HIP_ERROR_INVALID_TRANSPOSE_TYPE;
+    }
+    switch (tt) {
+        case DTFFT_HIP_TRANSPOSE_X_TO_Y: return p->z_slab ? DTFFT_HIP_ERROR_INVALID_TRANSPOSE_TYPE : p->t_xy.run(in, out, s);
+        case DTFFT_HIP_TRANSPOSE_Y_TO_X: return p->z_slab ? DTFFT_HIP_ERROR_INVALID_TRANSPOSE_TYPE : p->t_yx.run(in, out, s);
+        case DTFFT_HIP_TRANSPOSE_Y_TO_Z: return p->z_slab ? DTFFT_HIP_ERROR_INVALID_TRANSPOSE_TYPE : p->t_yz.run(in, out, s);
+        case DTFFT_HIP_TRANSPOSE_Z_TO_Y: return p->z_slab ? DTFFT_HIP_ERROR_INVALID_TRANSPOSE_TYPE : p->t_zy.run(in, out, s);
+        case DTFFT_HIP_TRANSPOSE_X_TO_Z: return p->z_slab ? p->t_xz.run(in, out, s) : DTFFT_HIP_ERROR_INVALID_TRANSPOSE_TYPE;
+        case DTFFT_HIP_TRANSPOSE_Z_TO_X: return p->z_slab ? p->t_zx.run(in, out, s) : DTFFT_HIP_ERROR_INVALID_TRANSPOSE_TYPE;
+        default: return DTFFT_HIP_ERROR_INVALID_TRANSPOSE_TYPE;
+    }
+}
+
+int32_t dtfft_hip_destroy(dtfft_hip_plan_t* plan) {
+    if (!plan) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    if (!*plan) return DTFFT_HIP_ERROR_PLAN_NOT_CREATED;
+    delete (plan_t*)*plan;
+    *plan = nullptr;
+    return DTFFT_HIP_SUCCESS;
+}
+
+int32_t dtfft_hip_get_local_sizes(dtfft_hip_plan_t plan, int32_t* in_starts, int32_t* in_counts, int32_t* out_starts,
+                                  int32_t* out_counts, size_t* alloc_size) {
+    auto* p = (plan_t*)plan;
+    if (!p) return DTFFT_HIP_ERROR_PLAN_NOT_CREATED;
+    if (!in_starts && !in_counts && !out_starts && !out_counts && !alloc_size) return DTFFT_HIP_ERROR_INVALID_USAGE;
+    const int nd = p->ndims;
+    // Fortran-order counts; the C API reports them reversed (include/dtfft.h:657-667)
+    int64_t fin[3] = {p->nx, p->ny, p->nz};
+    int64_t fout[3];
+    if (nd == 2) { fout[0] = p->ny; fout[1] = p->nc; fout[2] = 1; }
+    else { fout[0] = p->nz; fout[1] = p->nc; fout[2] = p->ny; }
+    for (int i = 0; i < nd; i++) {
+        if (in_starts) in_starts[i] = 0;
+        if (out_starts) out_starts[i] = 0;
+        if (in_counts) in_counts[i] = (int32_t)fin[nd - 1 - i];
+        if (out_counts) out_counts[i] = (int32_t)fout[nd - 1 - i];
+    }
+    if (alloc_size) *alloc_size = p->alloc_elems;
+    return DTFFT_HIP_SUCCESS;
+}
+
+int32_t dtfft_hip_get_alloc_size(dtfft_hip_plan_t plan, size_t* alloc_size) {
+    auto* p = (plan_t*)plan;
+    if (!p) return DTFFT_HIP_ERROR_PLAN_NOT_CREATED;
+    if (!alloc_size) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    *alloc_size = p->alloc_elems;
+    return DTFFT_HIP_SUCCESS;
+}
+
+int32_t dtfft_hip_get_element_size(dtfft_hip_plan_t plan, size_t* element_size) {
+    auto* p = (plan_t*)plan;
+    if (!p) return DTFFT_HIP_ERROR_PLAN_NOT_CREATED;
+    if (!element_size) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    *element_size = p->elem_size;
+    return DTFFT_HIP_SUCCESS;
+}
+
+int32_t dtfft_hip_get_alloc_bytes(dtfft_hip_plan_t plan, size_t* alloc_bytes) {
+    auto* p = (plan_t*)plan;
+    if (!p) return DTFFT_HIP_ERROR_PLAN_NOT_CREATED;
+    if (!alloc_bytes) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    *alloc_bytes = p->alloc_elems * p->elem_size;
+    return DTFFT_HIP_SUCCESS;
+}
+
+int32_t dtfft_hip_get_aux_bytes(dtfft_hip_plan_t plan, size_t* aux_bytes) {
+    return dtfft_hip_get_alloc_bytes(plan, aux_bytes);
+}
+
+int32_t dtfft_hip_get_z_slab_enabled(dtfft_hip_plan_t plan, int32_t* on) {
+    auto* p = (plan_t*)plan;
+    if (!p) return DTFFT_HIP_ERROR_PLAN_NOT_CREATED;
+    if (!on) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    *on = p->z_slab ? 1 : 0;
+    return DTFFT_HIP_SUCCESS;
+}
+
+int32_t dtfft_hip_plan_mem_alloc(dtfft_hip_plan_t plan, size_t alloc_bytes, void** ptr) {
+    if (!plan) return DTFFT_HIP_ERROR_PLAN_NOT_CREATED;
+    return dtfft_hip_mem_alloc(alloc_bytes, ptr);
+}
+
+int32_t dtfft_hip_plan_mem_free(dtfft_hip_plan_t plan, void* ptr) {
+    if (!plan) return DTFFT_HIP_ERROR_PLAN_NOT_CREATED;
+    return dtfft_hip_mem_free(ptr);
+}
+
+int32_t dtfft_hip_get_stream(dtfft_hip_plan_t plan, void** stream) {
+    auto* p = (plan_t*)plan;
+    if (!p) return DTFFT_HIP_ERROR_PLAN_NOT_CREATED;
+    if (!stream) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    *stream = (void*)p->stream;
+    return DTFFT_HIP_SUCCESS;
+}
+
+int32_t dtfft_hip_report(dtfft_hip_plan_t plan) {
+    auto* p = (plan_t*)plan;
+    if (!p) return DTFFT_HIP_ERROR_PLAN_NOT_CREATED;
+    std::printf("dtfft_hip plan: %dD %s %s, dims (X fastest) %lld x %lld x %lld, %s%s%s\n", p->ndims,
+                p->r2c ? "R2C" : "C2C", p->dbl ? "double" : "single", (long long)p->nx, (long long)p->ny,
+                (long long)p->nz, p->transpose_only ? "transpose-only" : "HIP executor",
+                p->z_slab ? ", Z-slab" : "", p->fused ? ", fused transposes" : "");
+    auto show = [](const char* name, const fft_stage& st) {
+        if (!st.k) return;
+        std::printf("  %-8s %s\n", name, st.cfg.expr().c_str());
+    };
+    show("fft_x", p->fx); show("fft_y", p->fy); show("fft_z", p->fz); show("fft_ycol", p->fy_cols);
+    show("xy+fft", p->u_xy); show("yz+fft", p->u_yz); show("zy+fft", p->u_zy); show("yx+fft", p->u_yx);
+    show("xz+fft", p->u_xz); show("zx+fft", p->u_zx);
+    return DTFFT_HIP_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,105 @@
+"""CPU checks of the C-ABI library: it loads, exports every symbol include/dtfft_hip.h
+declares, validates arguments the way the reference does, and compiles kernels with
+hipRTC for gfx950 without a GPU (compile-only mode).  No device work runs here."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "dtfft_hip.h")
+
+
+def header_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(dtfft_hip_\w+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def D():
+    import dtfft_amd
+    return dtfft_amd
+
+
+def test_library_exports_every_declared_symbol(D):
+    L = D.lib()
+    names = header_functions()
+    assert len(names) > 40
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_header(D):
+    from dtfft_amd._lib import SIGNATURES
+    assert set(header_functions()) == set(SIGNATURES)
+
+
+def test_error_strings(D):
+    L = D.lib()
+    assert L.dtfft_hip_get_error_string(0) == b"DTFFT_SUCCESS"
+    assert b"dimension" in L.dtfft_hip_get_error_string(4)
+    assert L.dtfft_hip_get_error_string(101).startswith(b"R2R")
+
+
+def _i32(v):
+    return (C.c_int32 * len(v))(*v)
+
+
+def test_argument_validation(D):
+    L = D.lib()
+    h = C.c_void_p()
+    # plan: invalid ndims / dims / precision / executor / R2C transpose-only
+    assert L.dtfft_hip_create_plan_c2c(4, _i32([2, 2, 2, 2]), 0, 0, 5, C.byref(h)) == 3
+    assert L.dtfft_hip_create_plan_c2c(3, _i32([2, 0, 2]), 0, 0, 5, C.byref(h)) == 4
+    assert L.dtfft_hip_create_plan_c2c(3, _i32([2, 2, 2]), 7, 0, 5, C.byref(h)) == 6
+    assert L.dtfft_hip_create_plan_c2c(3, _i32([2, 2, 2]), 0, 9, 5, C.byref(h)) == 7
+    assert L.dtfft_hip_create_plan_c2c(3, _i32([2, 2, 2]), 0, 0, 3, C.byref(h)) == 8
+    assert L.dtfft_hip_create_plan_r2c(3, _i32([2, 2, 2]), 0, 0, 0, C.byref(h)) == 13
+    # kernels: 2-step permutes are 3D only; pipelined kernels need neighbours
+    assert L.dtfft_hip_kernel_create(9, 2, _i32([4, 4]), 8, None, 0, 0, 0, C.byref(h)) == 603
+    assert L.dtfft_hip_kernel_create(6, 3, _i32([4, 4, 4]), 8, None, 0, 0, 0, C.byref(h)) == 607
+    assert L.dtfft_hip_kernel_create(7, 3, _i32([4, 4, 4]), 12, None, 0, 0, 0, C.byref(h)) == 607
+    # more than 2^31-1 elements per process
+    assert L.dtfft_hip_kernel_create(7, 3, _i32([2048, 2048, 1024]), 8, None, 0, 0, 0, C.byref(h)) == 605
+    # executor: R2R is not supported (as the cuFFT executor, error 101)
+    assert L.dtfft_hip_fft_create(1, 2, 0, 8, 8, 1, _i32([8]), _i32([8]), _i32([8]), None, 0, None,
+                                  C.byref(h)) == 101
+    # executor: a length with a prime factor > 13
+    assert L.dtfft_hip_fft_create(1, 0, 0, 17, 17, 1, _i32([17]), _i32([17]), _i32([17]), None, 0, None,
+                                  C.byref(h)) == 602
+    # destroying nothing
+    assert L.dtfft_hip_destroy(C.byref(C.c_void_p())) == 1
+
+
+def test_dummy_kernel_on_zero_dims(D):
+    """Zero dims make a dummy kernel that does nothing (src/dtfft_abstract_kernel.F90:233-237)."""
+    L = D.lib()
+    h = C.c_void_p()
+    assert L.dtfft_hip_kernel_create(7, 3, _i32([4, 0, 4]), 8, None, 0, 0, 0, C.byref(h)) == 0
+    assert L.dtfft_hip_kernel_execute(h, None, None, None, 0, 0) == 0
+    assert L.dtfft_hip_kernel_destroy(h) == 0
+
+
+def test_compile_only_plan_and_kernels(D, tmp_path):
+    """hipRTC compiles the kernels of a plan for gfx950 with no GPU present."""
+    L = D.lib()
+    if L.dtfft_hip_get_device_arch is None:  # pragma: no cover
+        pytest.skip()
+    arch = D.device_arch()
+    if arch.startswith("gfx950"):
+        pytest.skip("a GPU is present; compile-only behaviour is exercised on CPU hosts")
+    p = D.PlanC2C([12, 10, 6], D.Precision.DOUBLE, D.Effort.ESTIMATE, D.Executor.HIP)
+    ins, inc, outs, outc, alloc = p.local_sizes
+    assert inc == [12, 10, 6]
+    assert outc == [6, 12, 10]  # Z-pencil counts (nz, nx, ny), natural Fortran order
+    assert alloc == 720
+    assert p.element_size == 16
+    r = D.PlanR2C([16, 10, 6], D.Precision.SINGLE, D.Effort.ESTIMATE, D.Executor.HIP)
+    assert r.local_sizes[3] == [6, 9, 10]
+    assert r.alloc_size == max(16 * 60, 2 * 9 * 60)
+    assert r.element_size == 4
+    # execute needs a device
+    with pytest.raises(D.DtfftError) as e:
+        p.execute(1, 2, D.Execute.FORWARD)
+    assert e.value.code == 606

@@ -1,0 +1,99 @@
+"""GPU parity of the FFT executor plugin against the FP64 oracle.
+
+Bars (stated per result): FP64 plans within 1e-12 relative L2 of the oracle; FP32
+plans within 1e-6 relative L2 of the FP64 oracle evaluated on the same (FP32-rounded)
+input.  The oracle itself is pinned by tests/test_oracle_fft.py."""
+import numpy as np
+import pytest
+
+import oracle as O
+import oracle.plan as OP
+
+pytestmark = pytest.mark.gpu
+
+D = pytest.importorskip("dtfft_amd")
+
+TOL = {D.Precision.DOUBLE: 1e-12, D.Precision.SINGLE: 1e-6}
+LENGTHS = [2, 3, 4, 5, 7, 8, 11, 13, 16, 48, 60, 64, 84, 96, 105, 128, 160, 192, 210, 256, 320, 384, 512,
+           1024, 2048]
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def cdtype(prec):
+    return np.complex128 if prec == D.Precision.DOUBLE else np.complex64
+
+
+@pytest.mark.parametrize("prec", [D.Precision.SINGLE, D.Precision.DOUBLE])
+@pytest.mark.parametrize("n", LENGTHS)
+def test_c2c_1d(n, prec):
+    rng = np.random.default_rng(n)
+    hm = max(3, 4096 // n)
+    x = (rng.uniform(-1, 1, (hm, n)) + 1j * rng.uniform(-1, 1, (hm, n))).astype(cdtype(prec))
+    ex = D.FFTExecutor(1, D.FFTType.C2C, prec, n, n, hm, [n])
+    a = D.DeviceArray.from_numpy(x)
+    b = D.DeviceArray(x.nbytes)
+    x64 = x.astype(np.complex128)
+    for sign in (D.FFT_FORWARD, D.FFT_BACKWARD):
+        ex.execute(a, b, sign)
+        got = b.download(x.shape, x.dtype).astype(np.complex128)
+        want = O.fft_c2c(x64, sign)
+        assert rel(got, want) < TOL[prec], (n, sign, rel(got, want))
+    # in place
+    ex.execute(a, a, D.FFT_FORWARD)
+    assert rel(a.download(x.shape, x.dtype).astype(np.complex128), O.fft_c2c(x64)) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", [D.Precision.SINGLE, D.Precision.DOUBLE])
+@pytest.mark.parametrize("n", [2, 4, 6, 16, 60, 96, 210, 256, 512, 1024, 2048])
+def test_r2c_c2r_1d(n, prec):
+    rng = np.random.default_rng(n + 7)
+    hm = max(3, 2048 // n)
+    rdt = np.float64 if prec == D.Precision.DOUBLE else np.float32
+    nc = n // 2 + 1
+    x = rng.standard_normal((hm, n)).astype(rdt)
+    ex = D.FFTExecutor(1, D.FFTType.R2C, prec, n, nc, hm, [n], [n], [nc])
+    a = D.DeviceArray.from_numpy(x)
+    c = D.DeviceArray(hm * nc * np.dtype(cdtype(prec)).itemsize)
+    ex.execute(a, c, D.FFT_FORWARD)
+    got = c.download((hm, nc), cdtype(prec)).astype(np.complex128)
+    want = O.fft_r2c(x.astype(np.float64))
+    assert rel(got, want) < TOL[prec]
+    # C2R of a random half spectrum (imaginary parts of X_0 and X_{n/2} must be ignored)
+    h = (rng.standard_normal((hm, nc)) + 1j * rng.standard_normal((hm, nc))).astype(cdtype(prec))
+    c.upload(h)
+    r = D.DeviceArray(x.nbytes)
+    ex.execute(c, r, D.FFT_BACKWARD)
+    got = r.download((hm, n), rdt).astype(np.float64)
+    want = O.fft_c2r(h.astype(np.complex128), n)
+    assert rel(got, want) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", [D.Precision.SINGLE, D.Precision.DOUBLE])
+@pytest.mark.parametrize("shape", [(12, 10, 7), (3, 64, 48), (2, 105, 60)])
+def test_c2c_2d(shape, prec):
+    """fft_rank 2 (Z-slab executor): sizes (n_slow, n_fast) per batch."""
+    hm, n0, n1 = shape
+    rng = np.random.default_rng(sum(shape))
+    x = (rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape)).astype(cdtype(prec))
+    ex = D.FFTExecutor(2, D.FFTType.C2C, prec, n0 * n1, n0 * n1, hm, [n0, n1])
+    a = D.DeviceArray.from_numpy(x)
+    b = D.DeviceArray(x.nbytes)
+    ex.execute(a, b, D.FFT_FORWARD)
+    got = b.download(shape, x.dtype).astype(np.complex128)
+    want = OP._fft_2d(x.astype(np.complex128), O.FFT_FORWARD)
+    assert rel(got, want) < TOL[prec]
+
+
+def test_autotuned_stage_is_correct():
+    n, hm = 512, 2048
+    rng = np.random.default_rng(1)
+    x = (rng.uniform(-1, 1, (hm, n)) + 1j * rng.uniform(-1, 1, (hm, n))).astype(np.complex64)
+    ex = D.FFTExecutor(1, D.FFTType.C2C, D.Precision.SINGLE, n, n, hm, [n], effort=D.Effort.MEASURE)
+    a = D.DeviceArray.from_numpy(x)
+    b = D.DeviceArray(x.nbytes)
+    ex.execute(a, b, D.FFT_FORWARD)
+    got = b.download(x.shape, x.dtype).astype(np.complex128)
+    assert rel(got, O.fft_c2c(x.astype(np.complex128))) < 1e-6
