@@ -40,6 +40,9 @@ def parse():
     ap.add_argument("--effort", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--config", type=int, action="append", default=[],
+                    help="also run another workload: 2 transpose c128, 3 R2C, 4 batched, 5 pencil share")
+    ap.add_argument("--only-configs", action="store_true", help="skip the headline metric")
     return ap.parse_args()
 
 
@@ -73,6 +76,14 @@ def main():
               "(the multi-GPU exchange is out of scope here)", flush=True)
         sys.exit(2)
     import dtfft_amd as D
+
+    if args.config:
+        timer = Timer(D)
+        for cfg in args.config:
+            log(f"[bench] config {cfg}")
+            {2: config2, 3: config3, 4: config4, 5: config5}[cfg](D, args, timer)
+        if args.only_configs:
+            return
 
     n = args.n
     N = n ** 3
@@ -147,6 +158,12 @@ def main():
     value = algo_cycle / (ms_fused * 1e-3) / 1e9
     plain_gbs = algo_cycle / (ms_plain * 1e-3) / 1e9
 
+    # measured device-copy bandwidth: the box_copy kernel moving the same 1 GiB buffer
+    kc = D.Kernel(D.KernelType.PACK, [n, n, n], es, [[n, n, n, 0, 0]])
+    ms_copy = _time(D, timer, lambda: kc.execute(a, c, neighbor=0), args.steps, 2)
+    copy_gbs = algo_pass / (ms_copy * 1e-3) / 1e9
+    log(f"[bench] copy kernel {ms_copy:.3f} ms = {copy_gbs:.0f} GB/s")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(x, n, args.cpu_threads, algo_cycle)
@@ -169,6 +186,8 @@ def main():
                    "grid": [n, n, n], "algorithmic_bytes_per_cycle": algo_cycle},
         "hbm_frac": round(value / HBM_PEAK_GBS, 4),
         "cycles_per_s": round(1000.0 / ms_fused, 2),
+        "copy_kernel_gbs": round(copy_gbs, 1),
+        "frac_of_copy": round(value / copy_gbs, 4),
         "plain": {"value": round(plain_gbs, 2), "unit": "GB/s", "ms_per_step": round(ms_plain, 4),
                   "hbm_frac": round(plain_gbs / HBM_PEAK_GBS, 4)},
         "fused_over_plain": round(ms_plain / ms_fused, 3),
@@ -181,6 +200,129 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def _time(D, timer, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    D.synchronize()
+    timer.start()
+    for _ in range(steps):
+        fn()
+    ms = timer.stop_ms()
+    D.synchronize()
+    return ms / steps
+
+
+def _line(metric, value, unit, ms, args, workload, **extra):
+    d = {"metric": metric, "value": round(value, 2), "unit": unit, "n_gpus": 1, "steps": args.steps,
+         "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+         "vs_baseline": None, "data": "synthetic", "config": {"workload": workload},
+         "hbm_frac": round(value / HBM_PEAK_GBS, 4) if unit == "GB/s" else None}
+    d.update(extra)
+    print(json.dumps(d), flush=True)
+
+
+def config2(D, args, timer):
+    """512^3 complex128 transpose-only XYZ -> YZX -> ZXY -> XYZ (three permute launches)."""
+    n = 512
+    N = n ** 3
+    a = D.DeviceArray(N * 16)
+    b = D.DeviceArray(N * 16)
+    ks = [D.Kernel(D.KernelType.PERMUTE_FORWARD, d, 16, effort=D.Effort(args.effort))
+          for d in ((n, n, n), (n, n, n), (n, n, n))]
+
+    def cyc():
+        ks[0].execute(a, b)
+        ks[1].execute(b, a)
+        ks[2].execute(a, b)
+    ms = _time(D, timer, cyc, args.steps, args.warmup)
+    _line("transpose-only cycle GB/s, 512^3 c128", 6 * N * 16 / (ms * 1e-3) / 1e9, "GB/s", ms, args,
+          "512^3 complex128 XYZ->YZX->ZXY->XYZ, 3 permute launches", dtype="c128 (exact moves)")
+
+
+def config3(D, args, timer):
+    """R2C 1024x512x256 (X fastest) fwd+bwd, FP32 and the FP64 twin."""
+    nx, ny, nz = 1024, 512, 256
+    nc = nx // 2 + 1
+    for prec, rs in ((D.Precision.SINGLE, 4), (D.Precision.DOUBLE, 8)):
+        plan = D.PlanR2C([nx, ny, nz], prec, D.Effort(args.effort), D.Executor.HIP)
+        a = D.DeviceArray(plan.alloc_bytes)
+        b = D.DeviceArray(plan.alloc_bytes)
+        c = D.DeviceArray(plan.alloc_bytes)
+        aux = D.DeviceArray(plan.alloc_bytes)
+        rng = np.random.default_rng(3)
+        a.upload(rng.standard_normal(nx * ny * nz).astype(np.float32 if rs == 4 else np.float64))
+
+        def cyc():
+            plan.execute(a, b, D.Execute.FORWARD, aux)
+            plan.execute(b, c, D.Execute.BACKWARD, aux)
+        ms = _time(D, timer, cyc, args.steps, args.warmup)
+        R = nx * ny * nz * rs
+        Cb = nc * ny * nz * 2 * rs
+        algo = 2 * (R + 5 * Cb)
+        _line(f"R2C fwd+bwd cycle GB/s, 1024x512x256 {'fp32' if rs == 4 else 'fp64'}", algo / (ms * 1e-3) / 1e9,
+              "GB/s", ms, args, "3D R2C 1024x512x256, plain sequence (R2C X, Y, Z with permutes)",
+              dtype="f32" if rs == 4 else "f64", algorithmic_bytes=algo)
+
+
+def config4(D, args, timer):
+    """64 uneven 3D C2C complex64 grids: one persistent launch per stage, fwd+bwd as a
+    hipGraph, against 64 sequential plain per-grid plans."""
+    sizes = [48, 60, 64, 84, 96, 105, 128, 160, 192, 210, 256, 320, 384]
+    rng = np.random.default_rng(20250104)
+    grids = [list(map(int, rng.choice(sizes, 3))) for _ in range(64)]
+    t0 = time.time()
+    cy = D.Cycle(grids, D.Precision.SINGLE, D.Effort(args.effort))
+    log(f"[bench] cycle with 64 grids created in {time.time() - t0:.1f} s")
+    offs, total = cy.layout()
+    a, b, aux, c = (D.DeviceArray(total * 8) for _ in range(4))
+    a.upload(rng.uniform(-1, 1, 2 * total).astype(np.float32))
+    algo = sum(12 * g[0] * g[1] * g[2] * 8 for g in grids)
+    ms_b = _time(D, timer, lambda: cy.execute_graph(a, b, aux, c), args.steps, args.warmup)
+    D.set_config(enable_fused=False)
+    plans = [D.PlanC2C(g, D.Precision.SINGLE, D.Effort(args.effort), D.Executor.HIP) for g in grids]
+    D.set_config()
+
+    def plain():
+        for p, o in zip(plans, offs):
+            ob = o * 8
+            p.execute(a.ptr + ob, b.ptr + ob, D.Execute.FORWARD, aux.ptr + ob)
+            p.execute(b.ptr + ob, c.ptr + ob, D.Execute.BACKWARD, aux.ptr + ob)
+    ms_p = _time(D, timer, plain, max(3, args.steps // 4), 2)
+    _line("batched uneven fwd+bwd GB/s, 64 C2C complex64 grids", algo / (ms_b * 1e-3) / 1e9, "GB/s", ms_b, args,
+          "64 grids, axes drawn (seed 20250104) from {48..384}, persistent launch per stage + hipGraph",
+          dtype="c64", algorithmic_bytes=algo,
+          plain={"value": round(algo / (ms_p * 1e-3) / 1e9, 2), "ms_per_step": round(ms_p, 4),
+                 "form": "64 sequential plain plans (FFT + permute launches)"},
+          batched_over_plain=round(ms_p / ms_b, 3))
+
+
+def config5(D, args, timer):
+    """One-GPU share of a 4x4 pencil decomposition of 2048^3 complex64: Y-pencil
+    [2048, 512, 512] -> pack into 4 per-peer chunks -> loopback -> unpack + FFT along Z."""
+    P = 4
+    a = b = 2048
+    c = 512
+    res = {}
+    for fused in (True, False):
+        sh = D.ExchangeShare(a, b, c, P, 1, D.Precision.SINGLE, fused=fused, effort=D.Effort(args.effort))
+        S = sh.bytes
+        A = D.DeviceArray(S)
+        send = D.DeviceArray(S)
+        recv = D.DeviceArray(S)
+        B = D.DeviceArray(S)
+        tmp = None if fused else D.DeviceArray(S)
+        A.upload(np.random.default_rng(5).uniform(-1, 1, S // 4).astype(np.float32))
+        res[fused] = _time(D, timer, lambda: sh.run(A, send, recv, B, D.FFT_FORWARD, tmp=tmp),
+                           max(3, args.steps // 2), 2)
+        del A, send, recv, B, tmp
+    S = 2048 * 512 * 512 * 8
+    _line("pencil-share Y->Z stage GB/s, 2048x512x512 c64 (4 peers)", 6 * S / (res[True] * 1e-3) / 1e9, "GB/s",
+          res[True], args, "pack_forward x4 -> loopback copy -> unpack fused with FFT_Z (n=2048)",
+          dtype="c64", algorithmic_bytes=6 * S,
+          plain={"value": round(6 * S / (res[False] * 1e-3) / 1e9, 2), "ms_per_step": round(res[False], 4),
+                 "form": "pack x4, loopback, unpack x4, separate FFT_Z"})
 
 
 def cpu_baseline(x, n, threads, algo_cycle):

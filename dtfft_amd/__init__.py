@@ -320,6 +320,72 @@ class PlanR2C(_Plan):
     _create = "dtfft_hip_create_plan_r2c"
 
 
+class UnpackFFT:
+    """Receive side of a P > 1 transpose fused with the FFT along the aligned axis
+    (dtfft_hip_unpack_fft_*).  ``dims`` are the receive pencil's local dims (aligned
+    axis first), ``recv_rows`` the unpack neighbour data."""
+
+    def __init__(self, dims, recv_rows, precision: Precision, effort: Effort = Effort.ESTIMATE,
+                 stream: int | None = None):
+        self.h = C.c_void_p()
+        flat = [int(x) for r in recv_rows for x in r]
+        call("dtfft_hip_unpack_fft_create", len(dims), _i32arr(dims), _i32arr(flat), len(recv_rows),
+             int(precision), int(effort), C.c_void_p(stream), C.byref(self.h))
+
+    def execute(self, recv, out, sign: int = FFT_FORWARD) -> None:
+        call("dtfft_hip_unpack_fft_execute", self.h, C.c_void_p(_ptr(recv)), C.c_void_p(_ptr(out)), int(sign))
+
+    def destroy(self) -> None:
+        if self.h:
+            lib().dtfft_hip_unpack_fft_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class ExchangeShare:
+    """One GPU's share of a forward transpose on P peers (equal splits), with a loopback
+    copy in place of the exchange: PACK_FORWARD per peer into contiguous per-peer send
+    chunks, device copy of the send buffer into the receive buffer, then either UNPACK +
+    FFT along the aligned axis (``fused=False``) or the fused unpack+FFT (``fused=True``).
+    Pencil A = [a, b_me, c] -> B = [b, c, a_me]."""
+
+    def __init__(self, a: int, b: int, c: int, peers: int, me: int, precision: Precision, fused: bool = True,
+                 effort: Effort = Effort.ESTIMATE):
+        from . import pencil
+        if a % peers or b % peers:
+            raise ValueError("the loopback share needs equal splits")
+        self.a, self.b, self.c, self.P, self.me = a, b, c, peers, me
+        self.es = 16 if precision == Precision.DOUBLE else 8
+        send, recv, sc, rc = pencil.forward_exchange(a, b, c, peers, me)
+        self.send_rows, self.recv_rows = send, recv
+        self.bytes = sum(sc) * self.es
+        bme, ame = b // peers, a // peers
+        self.a_dims, self.b_dims = (a, bme, c), (b, c, ame)
+        self.pack = Kernel(KernelType.PACK_FORWARD, self.a_dims, self.es, send, effort)
+        self.fused = fused
+        if fused:
+            self.unpack_fft = UnpackFFT(self.b_dims, recv, precision, effort)
+        else:
+            self.unpack = Kernel(KernelType.UNPACK, self.b_dims, self.es, recv, effort)
+            self.fft = FFTExecutor(1, FFTType.C2C, precision, b, b, c * ame, [b], effort=effort)
+
+    def run(self, A, send, recv, B, sign: int = FFT_FORWARD, tmp=None) -> None:
+        for i in range(1, self.P + 1):
+            self.pack.execute(A, send, neighbor=i)
+        call("dtfft_hip_memcpy_d2d_async", C.c_void_p(_ptr(recv)), C.c_void_p(_ptr(send)),
+             C.c_size_t(self.bytes), None)
+        if self.fused:
+            self.unpack_fft.execute(recv, B, sign)
+        else:
+            self.unpack.execute(recv, tmp)
+            self.fft.execute(tmp, B, sign)
+
+
 class Cycle:
     """Batched fwd+bwd cycle over many 3D C2C grids (dims per grid in Fortran order)."""
 

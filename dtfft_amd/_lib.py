@@ -70,6 +70,9 @@ SIGNATURES = {
     "dtfft_hip_get_stream": (_i32, [_vp, _pvp]),
     "dtfft_hip_report": (_i32, [_vp]),
     "dtfft_hip_plan_execute_stage": (_i32, [_vp, _vp, _vp, _i32, _vp, _i32, _pi32]),
+    "dtfft_hip_unpack_fft_create": (_i32, [_i32, _pi32, _pi32, _i32, _i32, _i32, _vp, _pvp]),
+    "dtfft_hip_unpack_fft_execute": (_i32, [_vp, _vp, _vp, _i8]),
+    "dtfft_hip_unpack_fft_destroy": (_i32, [_vp]),
     "dtfft_hip_cycle_create": (_i32, [_i32, _pi32, _i32, _i32, _vp, _pvp]),
     "dtfft_hip_cycle_get_layout": (_i32, [_vp, _pi64, _psz]),
     "dtfft_hip_cycle_execute": (_i32, [_vp, _vp, _vp, _vp, _i32]),
@@ -77,6 +80,7 @@ SIGNATURES = {
     "dtfft_hip_cycle_destroy": (_i32, [_pvp]),
     "dtfft_hip_memcpy_h2d": (_i32, [_vp, _vp, _sz]),
     "dtfft_hip_memcpy_d2h": (_i32, [_vp, _vp, _sz]),
+    "dtfft_hip_memcpy_d2d_async": (_i32, [_vp, _vp, _sz, _vp]),
     "dtfft_hip_device_synchronize": (_i32, []),
     "dtfft_hip_stream_create": (_i32, [_pvp]),
     "dtfft_hip_stream_destroy": (_i32, [_vp]),

@@ -1,19 +1,283 @@
-// Batched cycle over many independent 3D C2C grids (no counterpart in the reference).
+// Batched cycle over many independent 3D C2C grids (no counterpart in the reference,
+// where every plan launches its own kernels).
+//
+// The grids are packed one after another in each buffer.  A direction is three
+// stages, each ONE persistent launch covering every grid:
+//   forward : FFT_X (in -> out), X->Y + FFT_Y (out -> aux), Y->Z + FFT_Z (aux -> out)
+//   backward: FFT_Z (in -> out), Z->Y + FFT_Y (out -> aux), Y->X + FFT_X (aux -> out)
+// i.e. the fused sequence of a single-rank plan, with the lengths of all grids compiled
+// into one kernel (one tile shape per distinct length) and a host-sorted work list of
+// (grid, tile) items walked by the resident blocks.  The forward+backward cycle is
+// captured once as a hipGraph and replayed.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <numeric>
+
 #include "internal.h"
+
+namespace dtfft_hip {
+
+struct grid_stage_h {  // mirrors dtfft::grid_stage
+    lines_desc d;
+    int64_t in_off, out_off;
+    const void* tw;
+    int32_t cls;
+    int32_t pad_;
+};
+
+struct stage_spec {  // per-grid geometry of one stage
+    int64_t n, nl, nb, in_ls, in_bs, in_ns, out_ls, out_bs, out_ns;
+};
+
+struct cycle_stage {
+    std::vector<fft_config> classes;
+    std::vector<std::shared_ptr<twiddles>> tws;
+    const rtc_kernel* k = nullptr;
+    int ntmax = 0;
+    void* d_gs = nullptr;
+    void* d_items = nullptr;
+    int nitems = 0;
+    unsigned grid = 0;
+    ~cycle_stage() {
+        if (d_gs) hipFree(d_gs);
+        if (d_items) hipFree(d_items);
+    }
+};
+
+struct cycle_t {
+    int ng = 0;
+    std::vector<std::array<int64_t, 3>> dims;  // (nx, ny, nz) per grid
+    std::vector<int64_t> off;
+    int64_t total = 0;
+    bool dbl = false;
+    int effort = 0;
+    hipStream_t stream = nullptr;
+    cycle_stage st[2][3];  // [fwd/bwd][stage]
+    // graph of fwd+bwd for one set of pointers
+    hipStream_t gstream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    void* gptr[4] = {nullptr, nullptr, nullptr, nullptr};
+    ~cycle_t() {
+        if (gexec) hipGraphExecDestroy(gexec);
+        if (graph) hipGraphDestroy(graph);
+        if (gstream) hipStreamDestroy(gstream);
+        if (ev_a) hipEventDestroy(ev_a);
+        if (ev_b) hipEventDestroy(ev_b);
+    }
+};
+
+static stage_spec spec_of(int dir, int s, const std::array<int64_t, 3>& g) {
+    const int64_t nx = g[0], ny = g[1], nz = g[2];
+    if (dir == 0) {
+        if (s == 0) return {nx, ny * nz, 1, nx, 0, 1, nx, 0, 1};                 // FFT_X
+        if (s == 1) return {ny, nx, nz, 1, nx * ny, nx, ny * nz, ny, 1};          // X->Y + FFT_Y
+        return {nz, ny, nx, 1, ny * nz, ny, nz * nx, nz, 1};                      // Y->Z + FFT_Z
+    }
+    // backward in the order whose strided reads stay one row apart (same result on one grid)
+    if (s == 0) return {nz, nx * ny, 1, nz, 0, 1, nz, 0, 1};                      // FFT_Z
+    if (s == 1) return {nx, nz, ny, 1, nz * nx, nz, nx * ny, nx, 1};              // Z->X + FFT_X
+    return {ny, nx, nz, 1, nx * ny, nx, 1, nx * ny, nx};                          // FFT_Y in place
+}
+
+static std::string tile_type(const fft_config& c) {
+    std::string s = "dtfft::fft_tile<";
+    s += c.dbl ? "double" : "float";
+    s += ", dtfft::sched<" + std::to_string(c.n);
+    for (int r : c.radices) s += ", " + std::to_string(r);
+    s += ">, " + std::to_string(c.tpl) + ", " + std::to_string(c.lpb) + ", " + (c.in_cols ? "true" : "false") +
+         ", " + (c.out_cols ? "true" : "false") + ", " + std::to_string(c.pad) + ">";
+    return s;
+}
+
+static int32_t build_stage(cycle_t* cy, int dir, int s) {
+    cycle_stage& cs = cy->st[dir][s];
+    std::vector<stage_spec> specs;
+    std::vector<int> cls_of(cy->ng);
+    for (int g = 0; g < cy->ng; g++) {
+        stage_spec sp = spec_of(dir, s, cy->dims[g]);
+        std::vector<int> rad;
+        if (!factorize((int)sp.n, rad) || sp.n < 2) {
+            set_last_error("batched cycle: unsupported length " + std::to_string(sp.n));
+            return DTFFT_HIP_ERROR_UNSUPPORTED_SIZE;
+        }
+        const bool in_cols = sp.in_ns != 1 && sp.in_ls == 1;
+        const bool out_cols = sp.out_ns != 1 && sp.out_ls == 1;
+        int found = -1;
+        for (size_t c = 0; c < cs.classes.size(); c++)
+            if (cs.classes[c].n == sp.n) found = (int)c;
+        if (found < 0) {
+            cs.classes.push_back(default_fft_config((int)sp.n, cy->dbl, in_cols, out_cols, in_cols && out_cols));
+            found = (int)cs.classes.size() - 1;
+        }
+        cls_of[g] = found;
+        specs.push_back(sp);
+    }
+    std::string expr = std::string("batched_fft_kernel<") + (cy->dbl ? "double" : "float") + ", ";
+    cs.ntmax = 0;
+    for (auto& c : cs.classes) cs.ntmax = std::max(cs.ntmax, c.tpl * c.lpb);
+    expr += std::to_string(cs.ntmax);
+    for (auto& c : cs.classes) expr += ", " + tile_type(c);
+    expr += ">";
+    TRY(rtc_get("fft", expr, &cs.k));
+    cs.tws.resize(cs.classes.size());
+    for (size_t c = 0; c < cs.classes.size(); c++)
+        TRY(get_twiddles(cs.classes[c].n, cs.classes[c].radices, cy->dbl, &cs.tws[c]));
+    // descriptors and the sorted work list
+    std::vector<grid_stage_h> gs(cy->ng);
+    struct item { int g, t; int64_t cost; };
+    std::vector<item> items;
+    for (int g = 0; g < cy->ng; g++) {
+        const stage_spec& sp = specs[g];
+        const fft_config& c = cs.classes[cls_of[g]];
+        grid_stage_h& h = gs[g];
+        h.d = make_lines(sp.nl, sp.nb, sp.in_ls, sp.in_bs, sp.in_ns, sp.out_ls, sp.out_bs, sp.out_ns, c.lpb);
+        h.in_off = cy->off[g];
+        h.out_off = cy->off[g];
+        h.tw = cs.tws[cls_of[g]]->ptr;
+        h.cls = cls_of[g];
+        h.pad_ = 0;
+        const int64_t cost = (int64_t)c.lpb * sp.n;
+        for (int64_t t = 0; t < h.d.ntiles; t++) items.push_back({g, (int)t, cost});
+    }
+    std::stable_sort(items.begin(), items.end(), [](const item& a, const item& b) { return a.cost > b.cost; });
+    cs.nitems = (int)items.size();
+    std::vector<int> flat(2 * items.size());
+    for (size_t i = 0; i < items.size(); i++) { flat[2 * i] = items[i].g; flat[2 * i + 1] = items[i].t; }
+    if (compile_only()) return DTFFT_HIP_SUCCESS;
+    HIP_TRY(hipMalloc(&cs.d_gs, sizeof(grid_stage_h) * gs.size()));
+    HIP_TRY(hipMemcpy(cs.d_gs, gs.data(), sizeof(grid_stage_h) * gs.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&cs.d_items, sizeof(int) * flat.size()));
+    HIP_TRY(hipMemcpy(cs.d_items, flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice));
+    int per_cu = 1;
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, cs.k->fn, cs.ntmax, 0) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    const int64_t resident = (int64_t)per_cu * std::max(1, device().num_cus);
+    cs.grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cs.nitems, resident));
+    return DTFFT_HIP_SUCCESS;
+}
+
+static int32_t run_stage(const cycle_t* cy, int dir, int s, const void* in, void* out, hipStream_t st) {
+    const cycle_stage& cs = cy->st[dir][s];
+    int inv = dir == 1 ? 1 : 0;
+    void* gsp = cs.d_gs;
+    void* itp = cs.d_items;
+    int nitems = cs.nitems;
+    void* args[] = {(void*)&in, &out, &gsp, &itp, &nitems, &inv};
+    return launch(cs.k, cs.grid, (unsigned)cs.ntmax, st, args);
+}
+
+static int32_t run_dir(const cycle_t* cy, int dir, void* in, void* out, void* aux, hipStream_t st) {
+    if (dir == 0) {  // in -> out -> aux -> out
+        TRY(run_stage(cy, 0, 0, in, out, st));
+        TRY(run_stage(cy, 0, 1, out, aux, st));
+        return run_stage(cy, 0, 2, aux, out, st);
+    }
+    // in -> aux -> out, then in place on out
+    TRY(run_stage(cy, 1, 0, in, aux, st));
+    TRY(run_stage(cy, 1, 1, aux, out, st));
+    return run_stage(cy, 1, 2, out, out, st);
+}
+
+}  // namespace dtfft_hip
+
+using namespace dtfft_hip;
 
 extern "C" {
 
-int32_t dtfft_hip_cycle_create(int32_t, const int32_t*, int32_t, int32_t, void*, dtfft_hip_cycle_t* cycle) {
-    if (cycle) *cycle = nullptr;
-    return DTFFT_HIP_ERROR_INVALID_USAGE;
+int32_t dtfft_hip_cycle_create(int32_t n_grids, const int32_t* dims, int32_t precision, int32_t effort, void* stream,
+                               dtfft_hip_cycle_t* cycle) {
+    if (!cycle || !dims) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    *cycle = nullptr;
+    if (n_grids <= 0) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    if (precision != DTFFT_HIP_SINGLE && precision != DTFFT_HIP_DOUBLE) return DTFFT_HIP_ERROR_INVALID_PRECISION;
+    if (effort < DTFFT_HIP_ESTIMATE || effort > DTFFT_HIP_EXHAUSTIVE) return DTFFT_HIP_ERROR_INVALID_EFFORT_FLAG;
+    auto cy = std::make_unique<cycle_t>();
+    cy->ng = n_grids;
+    cy->dbl = precision == DTFFT_HIP_DOUBLE;
+    cy->effort = effort;
+    cy->stream = (hipStream_t)stream;
+    int64_t off = 0;
+    for (int g = 0; g < n_grids; g++) {
+        // reverse order per grid, as dtfft_create_plan_c2c
+        std::array<int64_t, 3> d = {dims[3 * g + 2], dims[3 * g + 1], dims[3 * g + 0]};
+        for (auto v : d) if (v <= 0) return DTFFT_HIP_ERROR_INVALID_DIMENSION_SIZE;
+        const int64_t n = d[0] * d[1] * d[2];
+        if (n > INT32_MAX) return DTFFT_HIP_ERROR_INDEX_OVERFLOW;
+        cy->dims.push_back(d);
+        cy->off.push_back(off);
+        off += (n + 63) / 64 * 64;  // keep every grid 512-byte aligned
+    }
+    cy->total = off;
+    for (int dir = 0; dir < 2; dir++)
+        for (int s = 0; s < 3; s++) TRY(build_stage(cy.get(), dir, s));
+    *cycle = cy.release();
+    return DTFFT_HIP_SUCCESS;
 }
-int32_t dtfft_hip_cycle_get_layout(dtfft_hip_cycle_t, int64_t*, size_t*) { return DTFFT_HIP_ERROR_PLAN_NOT_CREATED; }
-int32_t dtfft_hip_cycle_execute(dtfft_hip_cycle_t, void*, void*, void*, int32_t) {
-    return DTFFT_HIP_ERROR_PLAN_NOT_CREATED;
+
+int32_t dtfft_hip_cycle_get_layout(dtfft_hip_cycle_t cycle, int64_t* offsets, size_t* alloc_size) {
+    auto* cy = (cycle_t*)cycle;
+    if (!cy) return DTFFT_HIP_ERROR_PLAN_NOT_CREATED;
+    if (offsets) for (int g = 0; g < cy->ng; g++) offsets[g] = cy->off[g];
+    if (alloc_size) *alloc_size = (size_t)cy->total;
+    return DTFFT_HIP_SUCCESS;
 }
-int32_t dtfft_hip_cycle_execute_graph(dtfft_hip_cycle_t, void*, void*, void*, void*) {
-    return DTFFT_HIP_ERROR_PLAN_NOT_CREATED;
+
+int32_t dtfft_hip_cycle_execute(dtfft_hip_cycle_t cycle, void* in, void* out, void* aux, int32_t execute_type) {
+    auto* cy = (cycle_t*)cycle;
+    if (!cy) return DTFFT_HIP_ERROR_PLAN_NOT_CREATED;
+    if (execute_type != DTFFT_HIP_EXECUTE_FORWARD && execute_type != DTFFT_HIP_EXECUTE_BACKWARD)
+        return DTFFT_HIP_ERROR_INVALID_EXECUTE_TYPE;
+    if (!in || !out || !aux) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    if (aux == in || aux == out) return DTFFT_HIP_ERROR_INVALID_AUX;
+    if (compile_only()) return DTFFT_HIP_ERROR_NO_DEVICE;
+    return run_dir(cy, execute_type == DTFFT_HIP_EXECUTE_FORWARD ? 0 : 1, in, out, aux, cy->stream);
 }
-int32_t dtfft_hip_cycle_destroy(dtfft_hip_cycle_t*) { return DTFFT_HIP_ERROR_PLAN_NOT_CREATED; }
+
+int32_t dtfft_hip_cycle_execute_graph(dtfft_hip_cycle_t cycle, void* in, void* out, void* aux, void* in2) {
+    auto* cy = (cycle_t*)cycle;
+    if (!cy) return DTFFT_HIP_ERROR_PLAN_NOT_CREATED;
+    if (!in || !out || !aux || !in2) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    if (aux == in || aux == out || aux == in2 || out == in || out == in2) return DTFFT_HIP_ERROR_INVALID_AUX;
+    if (compile_only()) return DTFFT_HIP_ERROR_NO_DEVICE;
+    void* p[4] = {in, out, aux, in2};
+    if (!cy->gexec || !std::equal(p, p + 4, cy->gptr)) {
+        if (!cy->gstream) {
+            HIP_TRY(hipStreamCreateWithFlags(&cy->gstream, hipStreamNonBlocking));
+            HIP_TRY(hipEventCreateWithFlags(&cy->ev_a, hipEventDisableTiming));
+            HIP_TRY(hipEventCreateWithFlags(&cy->ev_b, hipEventDisableTiming));
+        }
+        if (cy->gexec) { hipGraphExecDestroy(cy->gexec); cy->gexec = nullptr; }
+        if (cy->graph) { hipGraphDestroy(cy->graph); cy->graph = nullptr; }
+        HIP_TRY(hipStreamBeginCapture(cy->gstream, hipStreamCaptureModeThreadLocal));
+        int32_t rc = run_dir(cy, 0, in, out, aux, cy->gstream);
+        if (rc == DTFFT_HIP_SUCCESS) rc = run_dir(cy, 1, out, in2, aux, cy->gstream);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(cy->gstream, &g);
+        if (rc != DTFFT_HIP_SUCCESS) { if (g) hipGraphDestroy(g); return rc; }
+        if (e != hipSuccess) { set_last_error("hipStreamEndCapture failed"); return DTFFT_HIP_ERROR_HIP_RUNTIME; }
+        cy->graph = g;
+        HIP_TRY(hipGraphInstantiate(&cy->gexec, cy->graph, nullptr, nullptr, 0));
+        std::copy(p, p + 4, cy->gptr);
+    }
+    // order the replay after the caller's stream and the caller's stream after it
+    HIP_TRY(hipEventRecord(cy->ev_a, cy->stream));
+    HIP_TRY(hipStreamWaitEvent(cy->gstream, cy->ev_a, 0));
+    HIP_TRY(hipGraphLaunch(cy->gexec, cy->gstream));
+    HIP_TRY(hipEventRecord(cy->ev_b, cy->gstream));
+    HIP_TRY(hipStreamWaitEvent(cy->stream, cy->ev_b, 0));
+    return DTFFT_HIP_SUCCESS;
+}
+
+int32_t dtfft_hip_cycle_destroy(dtfft_hip_cycle_t* cycle) {
+    if (!cycle) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    if (!*cycle) return DTFFT_HIP_ERROR_PLAN_NOT_CREATED;
+    delete (cycle_t*)*cycle;
+    *cycle = nullptr;
+    return DTFFT_HIP_SUCCESS;
+}
 
 }  // extern "C"

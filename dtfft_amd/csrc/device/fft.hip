@@ -205,10 +205,17 @@ struct lines_desc {
     i64 out_ls, out_bs, out_ns;
     i64 ntl;             // ceil(nl / LPB)
     i64 ntiles;          // ntl * nb
+    i64 in_seg_stride;   // SEG > 0: input element n sits at (n / SEG) * in_seg_stride + (n % SEG) * in_ns
 };
 
-template <typename T, class S, int TPL, int LPB, bool IN_COLS, bool OUT_COLS, int PAD>
+// SEG > 0 reads the FFT axis in segments of SEG elements that lie in_seg_stride apart:
+// the per-peer chunks of a receive buffer, so that unpack and FFT are one pass.
+template <typename T, class S, int TPL, int LPB, bool IN_COLS, bool OUT_COLS, int PAD, int SEG = 0>
 struct fft_tile {
+    static __device__ __forceinline__ i64 in_elem(int n, const lines_desc& d) {
+        if constexpr (SEG > 0) return (i64)(n / SEG) * d.in_seg_stride + (i64)(n % SEG) * d.in_ns;
+        else return (i64)n * d.in_ns;
+    }
     typedef typename tw_of<T>::type tw_t;
     static constexpr int N = S::N;
     static constexpr int NP = S::NP;
@@ -236,20 +243,23 @@ struct fft_tile {
         constexpr int IT = (NB + TPL - 1) / TPL;
         constexpr bool EXACT = (IT * TPL == NB);
         const int tid = threadIdx.x;
+        // threads beyond NT exist when several tile shapes share one block (batched
+        // launch); they only take part in the barriers
+        const bool act = (int)blockDim.x == NT || tid < NT;
         int line, j;
         map<COLS>(tid, line, j);
-        const bool lv = (l0 + line) < d.nl;
+        const bool lv = act && (l0 + line) < d.nl;
         cx<T> v[IT][R];
 #pragma unroll
         for (int it = 0; it < IT; it++) {
             const int b = j + it * TPL;
-            if (EXACT || b < NB) {
+            if (act && (EXACT || b < NB)) {
 #pragma unroll
                 for (int r = 0; r < R; r++) {
                     const int n = b + r * NB;
                     if (SRC_G) {
                         cx<T> x = mk((T)0, (T)0);
-                        if (lv) x = gin[ibase + (i64)line * d.in_ls + (i64)n * d.in_ns];
+                        if (lv) x = gin[ibase + (i64)line * d.in_ls + in_elem(n, d)];
                         v[it][r] = inv ? conj(x) : x;
                     } else {
                         v[it][r] = lds[line * LD + n];
@@ -261,7 +271,7 @@ struct fft_tile {
 #pragma unroll
         for (int it = 0; it < IT; it++) {
             const int b = j + it * TPL;
-            if (EXACT || b < NB) {
+            if (act && (EXACT || b < NB)) {
                 const int k = b % L;
                 if (L > 1) {
 #pragma unroll
@@ -295,7 +305,7 @@ struct fft_tile {
                                                   i64 obase, const lines_desc& d, i64 l0, bool inv) {
         if constexpr (NP == 0 && p == 0 && FIRST_G != LAST_G) {
             // length-1 transform: only move the line between global memory and LDS
-            for (int line = threadIdx.x; line < LPB; line += NT) {
+            for (int line = threadIdx.x; line < LPB && (int)threadIdx.x < NT; line += NT) {
                 if (l0 + line >= d.nl) continue;
                 if (FIRST_G) {
                     cx<T> x = gin[ibase + (i64)line * d.in_ls];
@@ -312,11 +322,11 @@ struct fft_tile {
             if constexpr (first && FIRST_G && last) {
                 // single pass from global: stage the input through LDS so that in-place use
                 // is safe and the read and write mappings may differ
-                for (int e = threadIdx.x; e < LPB * N; e += NT) {
+                for (int e = threadIdx.x; e < LPB * N && (int)threadIdx.x < NT; e += NT) {
                     int line, n;
                     if (IN_COLS) { line = e % LPB; n = e / LPB; } else { line = e / N; n = e % N; }
                     cx<T> x = mk((T)0, (T)0);
-                    if (l0 + line < d.nl) x = gin[ibase + (i64)line * d.in_ls + (i64)n * d.in_ns];
+                    if (l0 + line < d.nl) x = gin[ibase + (i64)line * d.in_ls + in_elem(n, d)];
                     lds[line * LD + n] = inv ? conj(x) : x;
                 }
                 __syncthreads();
@@ -362,11 +372,11 @@ struct fft_tile {
 }  // namespace dtfft
 
 // Plain batched C2C launch: one tile per block (grid-stride over tiles).
-template <typename T, class S, int TPL, int LPB, bool IN_COLS, bool OUT_COLS, int PAD>
+template <typename T, class S, int TPL, int LPB, bool IN_COLS, bool OUT_COLS, int PAD, int SEG = 0>
 __global__ void __launch_bounds__(TPL * LPB)
 fft_c2c_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
                const typename tw_of<T>::type* __restrict__ tw, dtfft::lines_desc d, int inv) {
-    typedef dtfft::fft_tile<T, S, TPL, LPB, IN_COLS, OUT_COLS, PAD> F;
+    typedef dtfft::fft_tile<T, S, TPL, LPB, IN_COLS, OUT_COLS, PAD, SEG> F;
     __shared__ cx<T> lds[F::LDS_ELEMS];
     for (i64 t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
         F::run(in, out, tw, lds, d, t, inv != 0);
@@ -443,6 +453,52 @@ fft_c2r_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
         }
         __syncthreads();
         F::template passes<0, false, true>(in, out, tw, lds, ibase, obase, d, l0, true);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batched persistent launch: one stage of many independent grids.
+// Each grid has a descriptor (its lines in the packed buffers, its twiddles and the
+// index of its tile shape in the template list F...).  A fixed set of resident blocks
+// walks a host-sorted list of (grid, tile) work items; each item dispatches to the
+// tile shape of its grid's length.  blockDim is the largest shape; smaller shapes
+// leave the extra threads idle (they only join the barriers).
+// ---------------------------------------------------------------------------
+namespace dtfft {
+struct grid_stage {
+    lines_desc d;
+    i64 in_off, out_off;  // element offsets of the grid in the packed buffers
+    const void* tw;
+    int cls;
+    int pad_;
+};
+
+template <typename T, int I, class F0, class... Fs>
+__device__ __forceinline__ void dispatch(int cls, const cx<T>* in, cx<T>* out, const void* tw, cx<T>* lds,
+                                         const lines_desc& d, i64 tile, bool inv) {
+    if (cls == I) {
+        F0::run(in, out, (const typename tw_of<T>::type*)tw, lds, d, tile, inv);
+    } else if constexpr (sizeof...(Fs) > 0) {
+        dispatch<T, I + 1, Fs...>(cls, in, out, tw, lds, d, tile, inv);
+    }
+}
+
+template <class F0, class... Fs> struct max_lds {
+    static constexpr int v = F0::LDS_ELEMS > max_lds<Fs...>::v ? F0::LDS_ELEMS : max_lds<Fs...>::v;
+};
+template <class F0> struct max_lds<F0> { static constexpr int v = F0::LDS_ELEMS; };
+}  // namespace dtfft
+
+template <typename T, int NTMAX, class... F>
+__global__ void __launch_bounds__(NTMAX)
+batched_fft_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out, const dtfft::grid_stage* __restrict__ gs,
+                   const int2* __restrict__ items, int nitems, int inv) {
+    __shared__ cx<T> lds[dtfft::max_lds<F...>::v];
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const int2 w = items[it];
+        const dtfft::grid_stage& g = gs[w.x];
+        dtfft::dispatch<T, 0, F...>(g.cls, in + g.in_off, out + g.out_off, g.tw, lds, g.d, w.y, inv != 0);
         __syncthreads();
     }
 }

@@ -44,15 +44,17 @@ __global__ void __launch_bounds__(256) box_copy(void* __restrict__ vout, const v
         const i64 ib = d.din + i1 * d.is1 + i2 * d.is2;
         const i64 ob = d.dout + i1 * d.os1 + i2 * d.os2;
         const i64 x0 = (ch * 64 + lane) * VEC;
-        if (VEC > 1 && x0 + VEC <= d.e0) {
-            // aligned vector move (host guarantees offsets/strides are multiples of VEC)
-            typedef W vw __attribute__((ext_vector_type(VEC)));
-            *(vw*)(out + ob + x0) = *(const vw*)(in + ib + x0);
-        } else {
-#pragma unroll
-            for (int v = 0; v < VEC; v++)
-                if (x0 + v < d.e0) out[ob + x0 + v] = in[ib + x0 + v];
+        if constexpr (VEC > 1) {
+            if (x0 + VEC <= d.e0) {
+                // aligned vector move (host guarantees offsets/strides are multiples of VEC)
+                typedef W vw __attribute__((ext_vector_type(VEC)));
+                *(vw*)(out + ob + x0) = *(const vw*)(in + ib + x0);
+                continue;
+            }
         }
+#pragma unroll
+        for (int v = 0; v < VEC; v++)
+            if (x0 + v < d.e0) out[ob + x0 + v] = in[ib + x0 + v];
     }
 }
 

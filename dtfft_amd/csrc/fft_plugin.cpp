@@ -47,7 +47,9 @@ std::string fft_config::expr() const {
     for (int r : radices) s += ", " + std::to_string(r);
     s += ">, " + std::to_string(tpl) + ", " + std::to_string(lpb) + ", ";
     if (kind == 0) s += std::string(in_cols ? "true" : "false") + ", " + (out_cols ? "true" : "false") + ", ";
-    s += std::to_string(pad) + ">";
+    s += std::to_string(pad);
+    if (kind == 0 && seg > 0) s += ", " + std::to_string(seg);
+    s += ">";
     return s;
 }
 
@@ -173,7 +175,7 @@ int32_t get_rtwiddles(int n, bool dbl, std::shared_ptr<twiddles>* out) {
 // ---- stages ---------------------------------------------------------------------------
 lines_desc make_lines(int64_t nl, int64_t nb, int64_t in_ls, int64_t in_bs, int64_t in_ns, int64_t out_ls,
                       int64_t out_bs, int64_t out_ns, int lpb) {
-    lines_desc d;
+    lines_desc d{};
     d.nl = nl; d.nb = nb;
     d.in_ls = in_ls; d.in_bs = in_bs; d.in_ns = in_ns;
     d.out_ls = out_ls; d.out_bs = out_bs; d.out_ns = out_ns;

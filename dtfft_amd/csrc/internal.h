@@ -71,6 +71,7 @@ struct lines_desc {  // mirrors dtfft::lines_desc in device/fft.hip
     int64_t in_ls, in_bs, in_ns;
     int64_t out_ls, out_bs, out_ns;
     int64_t ntl, ntiles;
+    int64_t in_seg_stride;  // with fft_config::seg > 0: input chunks of `seg` elements this far apart
 };
 
 // Launch configuration of one batched 1D FFT kernel.
@@ -83,6 +84,7 @@ struct fft_config {
     bool in_cols = false, out_cols = false;
     bool dbl = false;
     int kind = 0;  // 0: C2C, 1: R2C (n = 2 * this->n real inputs), 2: C2R
+    int seg = 0;   // > 0: FFT axis read in segments of `seg` elements (fused unpack)
     std::string expr() const;
 };
 

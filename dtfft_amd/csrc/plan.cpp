@@ -58,7 +58,7 @@ struct plan_t {
 
     permute t_xy, t_yx, t_yz, t_zy, t_xz, t_zx;
     fft_stage fx, fx_c2r, fy, fz, fy_cols;           // plain stages
-    fft_stage u_xy, u_yz, u_zy, u_yx, u_xz, u_zx;    // fused stages
+    fft_stage u_xy, u_yz, u_zx, u_yx;                // fused stages (u_yx: 2D only)
 
     plan_t() = default;
     plan_t(const plan_t&) = delete;
@@ -130,21 +130,18 @@ static int32_t build(plan_t* p) {
         TRY(contig(p, nx, ny * nz, &p->fx));
     }
     TRY(contig(p, nz, nc * ny, &p->fz));
-    if (p->z_slab) {
+    if (p->z_slab || p->fused) {
         // Y along the strided axis of the X-pencil (second half of the 2D FFT), in place
         TRY(plan_fft_stage((int)ny, p->dbl, nc, nz, 1, nc * ny, nc, 1, nc * ny, nc, true, E, p->stream, &p->fy_cols));
-        if (p->fused) {
-            TRY(fused(p, nz, nc, ny, nc, nc * ny, nz, nz * nc, &p->u_xz));  // X -> Z + FFT_Z
-            TRY(fused(p, nx, nz, ny, nz * nc, nz, nc * ny, nc, &p->u_zx));  // Z -> X + FFT_X
-        }
-        return DTFFT_HIP_SUCCESS;
     }
-    TRY(contig(p, ny, nz * nc, &p->fy));
+    if (!p->z_slab) TRY(contig(p, ny, nz * nc, &p->fy));
     if (p->fused) {
+        // On one rank the generic and the Z-slab cycles end in the same Z-pencil, so the
+        // fused sequence picks the axis order whose strided reads stay short (one X or Y
+        // row apart): forward X, X->Y+Y, Y->Z+Z; backward Z, Z->X+X, Y in place.
         TRY(fused(p, ny, nc, nz, nc * ny, nc, ny * nz, ny, &p->u_xy));      // X -> Y + FFT_Y
         TRY(fused(p, nz, ny, nc, ny * nz, ny, nz * nc, nz, &p->u_yz));      // Y -> Z + FFT_Z
-        TRY(fused(p, ny, nz, nc, nz, nz * nc, ny, ny * nz, &p->u_zy));      // Z -> Y + FFT_Y
-        TRY(fused(p, nx, ny, nz, ny, ny * nz, nc, nc * ny, &p->u_yx));      // Y -> X + FFT_X
+        TRY(fused(p, nx, nz, ny, nz * nc, nz, nc * ny, nc, &p->u_zx));      // Z -> X + FFT_X
     }
     return DTFFT_HIP_SUCCESS;
 }
@@ -275,26 +272,15 @@ static int32_t exec_fused(plan_t* p, void* in, void* out, bool fwd, void* aux, s
         }
         return DTFFT_HIP_SUCCESS;
     }
-    if (p->z_slab) {
-        if (fwd) {
-            STEP(r, p->fx.run(in, aux, F, s));
-            STEP(r, p->fy_cols.run(aux, aux, F, s));
-            STEP(r, p->u_xz.run(aux, out, F, s));
-        } else {
-            STEP(r, p->fz.run(in, aux, B, s));
-            STEP(r, p->u_zx.run(aux, out, B, s));
-            STEP(r, p->fy_cols.run(out, out, B, s));
-        }
-        return DTFFT_HIP_SUCCESS;
-    }
+    // 3D, generic or Z-slab (same Z-pencil output on one rank)
     if (fwd) {
         STEP(r, p->fx.run(in, out, F, s));
         STEP(r, p->u_xy.run(out, aux, F, s));
         STEP(r, p->u_yz.run(aux, out, F, s));
     } else {
-        STEP(r, p->fz.run(in, out, B, s));
-        STEP(r, p->u_zy.run(out, aux, B, s));
-        STEP(r, p->u_yx.run(aux, out, B, s));
+        STEP(r, p->fz.run(in, aux, B, s));
+        STEP(r, p->u_zx.run(aux, out, B, s));
+        STEP(r, p->fy_cols.run(out, out, B, s));
     }
     return DTFFT_HIP_SUCCESS;
 }
@@ -481,8 +467,7 @@ int32_t dtfft_hip_report(dtfft_hip_plan_t plan) {
         std::printf("  %-8s %s\n", name, st.cfg.expr().c_str());
     };
     show("fft_x", p->fx); show("fft_y", p->fy); show("fft_z", p->fz); show("fft_ycol", p->fy_cols);
-    show("xy+fft", p->u_xy); show("yz+fft", p->u_yz); show("zy+fft", p->u_zy); show("yx+fft", p->u_yx);
-    show("xz+fft", p->u_xz); show("zx+fft", p->u_zx);
+    show("xy+fft", p->u_xy); show("yz+fft", p->u_yz); show("zx+fft", p->u_zx); show("yx+fft", p->u_yx);
     return DTFFT_HIP_SUCCESS;
 }
 

@@ -359,6 +359,10 @@ int32_t dtfft_hip_memcpy_d2h(void* dst, const void* src, size_t bytes) {
     HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return DTFFT_HIP_SUCCESS;
 }
+int32_t dtfft_hip_memcpy_d2d_async(void* dst, const void* src, size_t bytes, void* stream) {
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return DTFFT_HIP_SUCCESS;
+}
 int32_t dtfft_hip_device_synchronize(void) {
     HIP_TRY(hipDeviceSynchronize());
     return DTFFT_HIP_SUCCESS;

@@ -211,6 +211,17 @@ int32_t dtfft_hip_report(dtfft_hip_plan_t plan);
 int32_t dtfft_hip_plan_execute_stage(dtfft_hip_plan_t plan, void* in, void* out, int32_t execute_type,
                                      void* aux, int32_t stage, int32_t* n_stages);
 
+/* ---- receive side of a P > 1 transpose fused with the next FFT (no counterpart) -------
+ * Replaces KERNEL_UNPACK over all peers (src/dtfft_reshape_handle_generic.F90:606-636)
+ * followed by the FFT along the aligned axis, when the peers' chunks tile that axis in
+ * equal segments (neighbor_data rows (seg, ny, nz, din0 + i*chunk, i*seg)).  dims are the
+ * receive pencil's local dims (aligned axis first).  Returns INVALID_USAGE otherwise. */
+int32_t dtfft_hip_unpack_fft_create(int32_t ndims, const int32_t* dims, const int32_t* neighbor_data,
+                                    int32_t n_peers, int32_t precision, int32_t effort, void* stream,
+                                    void** handle);
+int32_t dtfft_hip_unpack_fft_execute(void* handle, const void* recv, void* out, int8_t sign);
+int32_t dtfft_hip_unpack_fft_destroy(void* handle);
+
 /* ---- batched cycle (no counterpart in the reference) ---------------------------------
  * n_grids independent 3D C2C grids (dims[3*g..3*g+2], reverse order) packed one after
  * another in one buffer; every stage of all grids runs in one persistent launch with
@@ -228,6 +239,9 @@ int32_t dtfft_hip_cycle_destroy(dtfft_hip_cycle_t* cycle);
 /* ---- plumbing used by tests and benchmarks ----------------------------------------- */
 int32_t dtfft_hip_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int32_t dtfft_hip_memcpy_d2h(void* dst, const void* src, size_t bytes);
+/* Asynchronous device-to-device copy on `stream` (the loopback that stands in for the
+   exchange on one GPU). */
+int32_t dtfft_hip_memcpy_d2d_async(void* dst, const void* src, size_t bytes, void* stream);
 int32_t dtfft_hip_device_synchronize(void);
 int32_t dtfft_hip_stream_create(void** stream);
 int32_t dtfft_hip_stream_destroy(void* stream);
