@@ -149,13 +149,16 @@ def main():
             for _ in range(args.steps):
                 fused.execute_stage(src, dst, et, s, aux)
             stage_ms[f"{name}{s}"] = timer.stop_ms() / args.steps
-    # stage 0 of each direction is the contiguous FFT; stages 1-2 the fused transpose+FFT
-    fused_keys = [k for k in stage_ms if not k.endswith("0")]
-    dom_ms = float(np.mean([stage_ms[k] for k in fused_keys]))
+    # Stage 0 of each direction is the FFT along contiguous rows.  fwd1 (X->Y), fwd2 (Y->Z)
+    # and bwd1 (Z->X) are the fused transpose+FFT kernel reading columns and writing rows,
+    # the dominant kernel; bwd2 is the FFT along columns in place.
+    dom_keys = ["fwd1", "fwd2", "bwd1"]
+    dom_ms = float(np.mean([stage_ms[k] for k in dom_keys]))
     achieved = algo_pass / (dom_ms * 1e-3) / 1e9
     log(f"[bench] stages (ms): {json.dumps({k: round(v, 4) for k, v in stage_ms.items()})}")
 
     value = algo_cycle / (ms_fused * 1e-3) / 1e9
+    traffic = _profiled_traffic() if n == 512 else None
     plain_gbs = algo_cycle / (ms_plain * 1e-3) / 1e9
 
     # measured device-copy bandwidth: the box_copy kernel moving the same 1 GiB buffer
@@ -194,12 +197,29 @@ def main():
         "roundtrip_rel_l2": rt_err,
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": TRAFFIC_FILE if traffic else None,
                      "kernel": "fused transpose+FFT pass (fft_c2c_kernel, column-in / row-out)",
                      "algorithmic_bytes_per_launch": algo_pass},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+TRAFFIC_FILE = "profiles/fft_512c64_pmc_traffic.md"
+
+
+def _profiled_traffic():
+    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (copy-kernel calibrated); None when the summary is absent."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TRAFFIC_FILE)
+    try:
+        for ln in open(path):
+            if ln.startswith("traffic_bytes_per_launch"):
+                return int(ln.split("=")[1])
+    except OSError:
+        return None
+    return None
 
 
 def _time(D, timer, fn, steps, warmup):
