@@ -34,14 +34,14 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--effort", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--config", type=int, action="append", default=[],
-                    help="also run another workload: 2 transpose c128, 3 R2C, 4 batched, 5 pencil share")
+                    help="also run another workload: 1 2D c128, 2 transpose c128, 3 R2C, 4 batched, 5 pencil share")
     ap.add_argument("--only-configs", action="store_true", help="skip the headline metric")
     return ap.parse_args()
 
@@ -65,6 +65,39 @@ class Timer:
         return float(ms.value)
 
 
+class StepTimer:
+    """One HIP event per step boundary on the default stream: the bracketed total over K
+    steps and each step's own duration (for the median and the minimum)."""
+
+    def __init__(self, D, steps):
+        self.D = D
+        self.ev = [C.c_void_p() for _ in range(steps + 1)]
+        for e in self.ev:
+            D.call("dtfft_hip_event_create", C.byref(e))
+
+    def run(self, fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        self.D.synchronize()
+        self.D.call("dtfft_hip_event_record", self.ev[0], None)
+        for i in range(steps):
+            fn()
+            self.D.call("dtfft_hip_event_record", self.ev[i + 1], None)
+        self.D.synchronize()
+        per = []
+        for i in range(steps):
+            ms = C.c_float()
+            self.D.call("dtfft_hip_event_elapsed_ms", self.ev[i], self.ev[i + 1], C.byref(ms))
+            per.append(float(ms.value))
+        tot = C.c_float()
+        self.D.call("dtfft_hip_event_elapsed_ms", self.ev[0], self.ev[steps], C.byref(tot))
+        return float(tot.value) / steps, float(np.median(per)), float(np.min(per))
+
+    def close(self):
+        for e in self.ev:
+            self.D.call("dtfft_hip_event_destroy", e)
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -81,7 +114,7 @@ def main():
         timer = Timer(D)
         for cfg in args.config:
             log(f"[bench] config {cfg}")
-            {2: config2, 3: config3, 4: config4, 5: config5}[cfg](D, args, timer)
+            {1: config1, 2: config2, 3: config3, 4: config4, 5: config5}[cfg](D, args, timer)
         if args.only_configs:
             return
 
@@ -117,24 +150,18 @@ def main():
         plan.execute(a, b, D.Execute.FORWARD, aux)
         plan.execute(b, c, D.Execute.BACKWARD, aux)
 
-    def time_cycles(plan, steps, warmup):
-        for _ in range(warmup):
-            cycle(plan)
-        D.synchronize()
-        timer.start()
-        for _ in range(steps):
-            cycle(plan)
-        ms = timer.stop_ms()
-        D.synchronize()
-        return ms / steps
+    steps_timer = StepTimer(D, args.steps)
 
-    ms_fused = time_cycles(fused, args.steps, args.warmup)
-    log(f"[bench] fused cycle {ms_fused:.3f} ms")
+    def time_cycles(plan, steps, warmup):
+        return steps_timer.run(lambda: cycle(plan), steps, warmup)
+
+    ms_fused, med_fused, min_fused = time_cycles(fused, args.steps, args.warmup)
+    log(f"[bench] fused cycle {ms_fused:.3f} ms (median {med_fused:.3f}, min {min_fused:.3f})")
     back = c.download(N, np.complex64)
     rt_err = float(np.linalg.norm((back / np.float32(N)).astype(np.complex128) - x) / np.linalg.norm(x))
     del back
-    ms_plain = time_cycles(plain, args.steps, args.warmup)
-    log(f"[bench] plain cycle {ms_plain:.3f} ms")
+    ms_plain, med_plain, min_plain = time_cycles(plain, args.steps, args.warmup)
+    log(f"[bench] plain cycle {ms_plain:.3f} ms (median {med_plain:.3f}, min {min_plain:.3f})")
 
     # per-launch durations of the fused sequence (dominant kernel for the roofline)
     stage_ms = {}
@@ -179,6 +206,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms_fused, 4),
+        "ms_median": round(med_fused, 4),
+        "ms_min": round(min_fused, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -192,6 +221,7 @@ def main():
         "copy_kernel_gbs": round(copy_gbs, 1),
         "frac_of_copy": round(value / copy_gbs, 4),
         "plain": {"value": round(plain_gbs, 2), "unit": "GB/s", "ms_per_step": round(ms_plain, 4),
+                  "ms_median": round(med_plain, 4), "ms_min": round(min_plain, 4),
                   "hbm_frac": round(plain_gbs / HBM_PEAK_GBS, 4)},
         "fused_over_plain": round(ms_plain / ms_fused, 3),
         "roundtrip_rel_l2": rt_err,
@@ -241,6 +271,33 @@ def _line(metric, value, unit, ms, args, workload, **extra):
          "hbm_frac": round(value / HBM_PEAK_GBS, 4) if unit == "GB/s" else None}
     d.update(extra)
     print(json.dumps(d), flush=True)
+
+
+def config1(D, args, timer):
+    """2D 256x256 C2C complex128 on one rank (the reference's execute_2d order):
+    FFT along X, X->Y permute, FFT along Y, and back; a launch-bound correctness case."""
+    n0 = n1 = 256
+    N = n0 * n1
+    rng = np.random.default_rng(1)
+    x = (rng.uniform(-1, 1, N) + 1j * rng.uniform(-1, 1, N)).astype(np.complex128)
+    algo = 2 * 2 * 2 * N * 16
+    for fused in (True, False):
+        D.set_config(enable_fused=fused)
+        plan = D.PlanC2C([n0, n1], D.Precision.DOUBLE, D.Effort(args.effort), D.Executor.HIP)
+        D.set_config()
+        a = D.DeviceArray.from_numpy(x)
+        b, c, aux = D.DeviceArray(N * 16), D.DeviceArray(N * 16), D.DeviceArray(N * 16)
+
+        def cyc():
+            plan.execute(a, b, D.Execute.FORWARD, aux)
+            plan.execute(b, c, D.Execute.BACKWARD, aux)
+        ms = _time(D, timer, cyc, args.steps, args.warmup)
+        back = c.download(N, np.complex128) / N
+        err = float(np.linalg.norm(back - x) / np.linalg.norm(x))
+        _line(f"2D C2C fwd+bwd cycle GB/s, 256x256 c128 ({'fused' if fused else 'plain'})",
+              algo / (ms * 1e-3) / 1e9, "GB/s", ms, args,
+              "2D C2C 256x256 complex128, one rank, fwd+bwd", dtype="c128", algorithmic_bytes=algo,
+              roundtrip_rel_l2=err, cycles_per_s=round(1000.0 / ms, 1))
 
 
 def config2(D, args, timer):

@@ -1,17 +1,21 @@
 // Batched cycle over many independent 3D C2C grids (no counterpart in the reference,
 // where every plan launches its own kernels).
 //
-// The grids are packed one after another in each buffer.  A direction is three
-// stages, each ONE persistent launch covering every grid:
+// The grids are packed one after another in each buffer.  A direction is three stages,
+// the fused sequence of a single-rank plan:
 //   forward : FFT_X (in -> out), X->Y + FFT_Y (out -> aux), Y->Z + FFT_Z (aux -> out)
-//   backward: FFT_Z (in -> out), Z->Y + FFT_Y (out -> aux), Y->X + FFT_X (aux -> out)
-// i.e. the fused sequence of a single-rank plan, with the lengths of all grids compiled
-// into one kernel (one tile shape per distinct length) and a host-sorted work list of
-// (grid, tile) items walked by the resident blocks.  The forward+backward cycle is
-// captured once as a hipGraph and replayed.
+//   backward: FFT_Z (in -> aux), Z->X + FFT_X (aux -> out), FFT_Y in place (out)
+// Within a stage the grids are grouped by the length of the stage's axis.  Each group is
+// one persistent launch of a kernel compiled for that length alone (so its register budget
+// is that of a single-length kernel), walking a work list of (grid, tile) items.  The groups
+// of a stage are independent: they are spread over side streams, forked from and joined
+// back to the caller's stream with events, so that small groups overlap.  The
+// forward+backward cycle is captured once as a hipGraph (the forks become parallel
+// branches) and replayed.
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <memory>
 #include <numeric>
 
 #include "internal.h"
@@ -30,18 +34,31 @@ struct stage_spec {  // per-grid geometry of one stage
     int64_t n, nl, nb, in_ls, in_bs, in_ns, out_ls, out_bs, out_ns;
 };
 
-struct cycle_stage {
-    std::vector<fft_config> classes;
-    std::vector<std::shared_ptr<twiddles>> tws;
+// One launch: every grid of a stage whose axis has this length.
+struct length_group {
+    fft_config cfg;
+    std::shared_ptr<twiddles> tw;
     const rtc_kernel* k = nullptr;
-    int ntmax = 0;
     void* d_gs = nullptr;
     void* d_items = nullptr;
     int nitems = 0;
-    unsigned grid = 0;
-    ~cycle_stage() {
+    unsigned grid = 0, block = 0;
+    int64_t work = 0;  // elements moved, for ordering the launches
+    ~length_group() {
         if (d_gs) hipFree(d_gs);
         if (d_items) hipFree(d_items);
+    }
+};
+
+static constexpr int kLanes = 4;  // the caller's stream + 3 side streams
+
+struct cycle_stage {
+    std::vector<std::unique_ptr<length_group>> groups;  // largest first
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[kLanes] = {};
+    ~cycle_stage() {
+        if (fork) hipEventDestroy(fork);
+        for (auto& e : join) if (e) hipEventDestroy(e);
     }
 };
 
@@ -53,6 +70,7 @@ struct cycle_t {
     bool dbl = false;
     int effort = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side[kLanes - 1] = {};
     cycle_stage st[2][3];  // [fwd/bwd][stage]
     // graph of fwd+bwd for one set of pointers
     hipStream_t gstream = nullptr;
@@ -64,6 +82,7 @@ struct cycle_t {
         if (gexec) hipGraphExecDestroy(gexec);
         if (graph) hipGraphDestroy(graph);
         if (gstream) hipStreamDestroy(gstream);
+        for (auto& x : side) if (x) hipStreamDestroy(x);
         if (ev_a) hipEventDestroy(ev_a);
         if (ev_b) hipEventDestroy(ev_b);
     }
@@ -94,80 +113,106 @@ static std::string tile_type(const fft_config& c) {
 
 static int32_t build_stage(cycle_t* cy, int dir, int s) {
     cycle_stage& cs = cy->st[dir][s];
-    std::vector<stage_spec> specs;
-    std::vector<int> cls_of(cy->ng);
+    std::vector<stage_spec> specs(cy->ng);
+    std::vector<int> group_of(cy->ng, -1);
+    std::vector<int64_t> lengths;
     for (int g = 0; g < cy->ng; g++) {
-        stage_spec sp = spec_of(dir, s, cy->dims[g]);
+        specs[g] = spec_of(dir, s, cy->dims[g]);
+        const stage_spec& sp = specs[g];
         std::vector<int> rad;
         if (!factorize((int)sp.n, rad) || sp.n < 2) {
             set_last_error("batched cycle: unsupported length " + std::to_string(sp.n));
             return DTFFT_HIP_ERROR_UNSUPPORTED_SIZE;
         }
+        auto it = std::find(lengths.begin(), lengths.end(), sp.n);
+        if (it == lengths.end()) { lengths.push_back(sp.n); it = lengths.end() - 1; }
+        group_of[g] = (int)(it - lengths.begin());
+    }
+    // every grid of a stage has the same layout kind (rows or columns on each side)
+    std::vector<std::unique_ptr<length_group>> groups;
+    for (size_t c = 0; c < lengths.size(); c++) {
+        auto grp = std::make_unique<length_group>();
+        int g0 = (int)(std::find(group_of.begin(), group_of.end(), (int)c) - group_of.begin());
+        const stage_spec& sp = specs[g0];
         const bool in_cols = sp.in_ns != 1 && sp.in_ls == 1;
         const bool out_cols = sp.out_ns != 1 && sp.out_ls == 1;
-        int found = -1;
-        for (size_t c = 0; c < cs.classes.size(); c++)
-            if (cs.classes[c].n == sp.n) found = (int)c;
-        if (found < 0) {
-            cs.classes.push_back(default_fft_config((int)sp.n, cy->dbl, in_cols, out_cols, in_cols && out_cols));
-            found = (int)cs.classes.size() - 1;
+        grp->cfg = default_fft_config((int)sp.n, cy->dbl, in_cols, out_cols, in_cols && out_cols);
+        const fft_config& cf = grp->cfg;
+        grp->block = (unsigned)(cf.tpl * cf.lpb);
+        const std::string expr = std::string("batched_fft_kernel<") + (cy->dbl ? "double" : "float") + ", " +
+                                 std::to_string(grp->block) + ", " + tile_type(cf) + ">";
+        TRY(rtc_get("fft", expr, &grp->k));
+        TRY(get_twiddles(cf.n, cf.radices, cy->dbl, &grp->tw));
+        std::vector<grid_stage_h> gs;
+        std::vector<int> flat;
+        for (int g = 0; g < cy->ng; g++) {
+            if (group_of[g] != (int)c) continue;
+            const stage_spec& q = specs[g];
+            grid_stage_h h{};
+            h.d = make_lines(q.nl, q.nb, q.in_ls, q.in_bs, q.in_ns, q.out_ls, q.out_bs, q.out_ns, cf.lpb);
+            h.in_off = cy->off[g];
+            h.out_off = cy->off[g];
+            h.tw = grp->tw->ptr;
+            h.cls = 0;
+            const int idx = (int)gs.size();
+            for (int64_t t = 0; t < h.d.ntiles; t++) { flat.push_back(idx); flat.push_back((int)t); }
+            grp->work += q.n * q.nl * q.nb;
+            gs.push_back(h);
         }
-        cls_of[g] = found;
-        specs.push_back(sp);
+        grp->nitems = (int)(flat.size() / 2);
+        if (!compile_only()) {
+            HIP_TRY(hipMalloc(&grp->d_gs, sizeof(grid_stage_h) * gs.size()));
+            HIP_TRY(hipMemcpy(grp->d_gs, gs.data(), sizeof(grid_stage_h) * gs.size(), hipMemcpyHostToDevice));
+            HIP_TRY(hipMalloc(&grp->d_items, sizeof(int) * flat.size()));
+            HIP_TRY(hipMemcpy(grp->d_items, flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice));
+            int per_cu = 1;
+            if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, grp->k->fn, (int)grp->block, 0) !=
+                    hipSuccess || per_cu < 1)
+                per_cu = 1;
+            const int64_t resident = (int64_t)per_cu * std::max(1, device().num_cus);
+            grp->grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(grp->nitems, resident));
+        }
+        groups.push_back(std::move(grp));
     }
-    std::string expr = std::string("batched_fft_kernel<") + (cy->dbl ? "double" : "float") + ", ";
-    cs.ntmax = 0;
-    for (auto& c : cs.classes) cs.ntmax = std::max(cs.ntmax, c.tpl * c.lpb);
-    expr += std::to_string(cs.ntmax);
-    for (auto& c : cs.classes) expr += ", " + tile_type(c);
-    expr += ">";
-    TRY(rtc_get("fft", expr, &cs.k));
-    cs.tws.resize(cs.classes.size());
-    for (size_t c = 0; c < cs.classes.size(); c++)
-        TRY(get_twiddles(cs.classes[c].n, cs.classes[c].radices, cy->dbl, &cs.tws[c]));
-    // descriptors and the sorted work list
-    std::vector<grid_stage_h> gs(cy->ng);
-    struct item { int g, t; int64_t cost; };
-    std::vector<item> items;
-    for (int g = 0; g < cy->ng; g++) {
-        const stage_spec& sp = specs[g];
-        const fft_config& c = cs.classes[cls_of[g]];
-        grid_stage_h& h = gs[g];
-        h.d = make_lines(sp.nl, sp.nb, sp.in_ls, sp.in_bs, sp.in_ns, sp.out_ls, sp.out_bs, sp.out_ns, c.lpb);
-        h.in_off = cy->off[g];
-        h.out_off = cy->off[g];
-        h.tw = cs.tws[cls_of[g]]->ptr;
-        h.cls = cls_of[g];
-        h.pad_ = 0;
-        const int64_t cost = (int64_t)c.lpb * sp.n;
-        for (int64_t t = 0; t < h.d.ntiles; t++) items.push_back({g, (int)t, cost});
+    std::stable_sort(groups.begin(), groups.end(),
+                     [](const std::unique_ptr<length_group>& a, const std::unique_ptr<length_group>& b) {
+                         return a->work > b->work;
+                     });
+    cs.groups = std::move(groups);
+    if (!compile_only() && cs.groups.size() > 1) {
+        HIP_TRY(hipEventCreateWithFlags(&cs.fork, hipEventDisableTiming));
+        for (auto& e : cs.join) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
-    std::stable_sort(items.begin(), items.end(), [](const item& a, const item& b) { return a.cost > b.cost; });
-    cs.nitems = (int)items.size();
-    std::vector<int> flat(2 * items.size());
-    for (size_t i = 0; i < items.size(); i++) { flat[2 * i] = items[i].g; flat[2 * i + 1] = items[i].t; }
-    if (compile_only()) return DTFFT_HIP_SUCCESS;
-    HIP_TRY(hipMalloc(&cs.d_gs, sizeof(grid_stage_h) * gs.size()));
-    HIP_TRY(hipMemcpy(cs.d_gs, gs.data(), sizeof(grid_stage_h) * gs.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMalloc(&cs.d_items, sizeof(int) * flat.size()));
-    HIP_TRY(hipMemcpy(cs.d_items, flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice));
-    int per_cu = 1;
-    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, cs.k->fn, cs.ntmax, 0) != hipSuccess ||
-        per_cu < 1)
-        per_cu = 1;
-    const int64_t resident = (int64_t)per_cu * std::max(1, device().num_cus);
-    cs.grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cs.nitems, resident));
     return DTFFT_HIP_SUCCESS;
 }
 
+static int32_t launch_group(const length_group& grp, int inv, const void* in, void* out, hipStream_t st) {
+    void* gsp = grp.d_gs;
+    void* itp = grp.d_items;
+    int nitems = grp.nitems;
+    void* args[] = {(void*)&in, &out, &gsp, &itp, &nitems, &inv};
+    return launch(grp.k, grp.grid, grp.block, st, args);
+}
+
+// One stage: group i runs on lane i % kLanes (lane 0 is the caller's stream); the side
+// lanes wait for the stage's fork event and the caller's stream waits for their joins.
 static int32_t run_stage(const cycle_t* cy, int dir, int s, const void* in, void* out, hipStream_t st) {
     const cycle_stage& cs = cy->st[dir][s];
-    int inv = dir == 1 ? 1 : 0;
-    void* gsp = cs.d_gs;
-    void* itp = cs.d_items;
-    int nitems = cs.nitems;
-    void* args[] = {(void*)&in, &out, &gsp, &itp, &nitems, &inv};
-    return launch(cs.k, cs.grid, (unsigned)cs.ntmax, st, args);
+    const int inv = dir == 1 ? 1 : 0;
+    const int ng = (int)cs.groups.size();
+    if (ng == 1) return launch_group(*cs.groups[0], inv, in, out, st);
+    const int lanes = std::min(ng, kLanes);
+    HIP_TRY(hipEventRecord(cs.fork, st));
+    for (int l = 1; l < lanes; l++) HIP_TRY(hipStreamWaitEvent(cy->side[l - 1], cs.fork, 0));
+    for (int i = 0; i < ng; i++) {
+        const int l = i % lanes;
+        TRY(launch_group(*cs.groups[i], inv, in, out, l == 0 ? st : cy->side[l - 1]));
+    }
+    for (int l = 1; l < lanes; l++) {
+        HIP_TRY(hipEventRecord(cs.join[l], cy->side[l - 1]));
+        HIP_TRY(hipStreamWaitEvent(st, cs.join[l], 0));
+    }
+    return DTFFT_HIP_SUCCESS;
 }
 
 static int32_t run_dir(const cycle_t* cy, int dir, void* in, void* out, void* aux, hipStream_t st) {
@@ -212,6 +257,8 @@ int32_t dtfft_hip_cycle_create(int32_t n_grids, const int32_t* dims, int32_t pre
         off += (n + 63) / 64 * 64;  // keep every grid 512-byte aligned
     }
     cy->total = off;
+    if (!compile_only())
+        for (auto& x : cy->side) HIP_TRY(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
     for (int dir = 0; dir < 2; dir++)
         for (int s = 0; s < 3; s++) TRY(build_stage(cy.get(), dir, s));
     *cycle = cy.release();
