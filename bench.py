@@ -41,7 +41,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--config", type=int, action="append", default=[],
-                    help="also run another workload: 1 2D c128, 2 transpose c128, 3 R2C, 4 batched, 5 pencil share")
+                    help="also run another workload: 1 2D c128, 2 transpose c128, 3 R2C, 4 batched, 5 pencil share, "
+                         "6 R2R")
     ap.add_argument("--only-configs", action="store_true", help="skip the headline metric")
     return ap.parse_args()
 
@@ -114,7 +115,7 @@ def main():
         timer = Timer(D)
         for cfg in args.config:
             log(f"[bench] config {cfg}")
-            {1: config1, 2: config2, 3: config3, 4: config4, 5: config5}[cfg](D, args, timer)
+            {1: config1, 2: config2, 3: config3, 4: config4, 5: config5, 6: config6}[cfg](D, args, timer)
         if args.only_configs:
             return
 
@@ -229,7 +230,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": TRAFFIC_FILE if traffic else None,
-                     "kernel": "fused transpose+FFT pass (fft_c2c_kernel, column-in / row-out)",
+                     "kernel": "fused transpose+FFT pass, column-in / row-out (fft_cols_dma_kernel: LDS "
+                               "double buffer filled by global_load_lds_dwordx4)",
                      "algorithmic_bytes_per_launch": algo_pass},
         "cpu_baseline": cpu,
     }
@@ -300,6 +302,25 @@ def config1(D, args, timer):
               roundtrip_rel_l2=err, cycles_per_s=round(1000.0 / ms, 1))
 
 
+def config6(D, args, timer):
+    """3D R2R 512^3 FP32: DCT-2 along every axis forward, DCT-3 (its inverse) backward; the
+    reference's plain sequence with R2R stages (pre-process, C2C, post-process each)."""
+    n = 512
+    N = n ** 3
+    plan = D.PlanR2R([n, n, n], [5, 5, 5], D.Precision.SINGLE, D.Effort(args.effort), D.Executor.HIP)
+    a, b, c, aux = (D.DeviceArray(N * 4) for _ in range(4))
+    a.upload(np.random.default_rng(6).standard_normal(N).astype(np.float32))
+
+    def cyc():
+        plan.execute(a, b, D.Execute.FORWARD, aux)
+        plan.execute(b, c, D.Execute.BACKWARD, aux)
+    ms = _time(D, timer, cyc, args.steps, args.warmup)
+    algo = 12 * N * 4
+    _line("R2R fwd+bwd cycle GB/s, 512^3 fp32 DCT-2/DCT-3", algo / (ms * 1e-3) / 1e9, "GB/s", ms, args,
+          "3D R2R 512^3 float32, DCT-2 on X, Y, Z forward, DCT-3 backward, with permutes",
+          dtype="f32", algorithmic_bytes=algo)
+
+
 def config2(D, args, timer):
     """512^3 complex128 transpose-only XYZ -> YZX -> ZXY -> XYZ (three permute launches)."""
     n = 512
@@ -319,28 +340,40 @@ def config2(D, args, timer):
 
 
 def config3(D, args, timer):
-    """R2C 1024x512x256 (X fastest) fwd+bwd, FP32 and the FP64 twin."""
+    """R2C 1024x512x256 (X fastest) fwd+bwd, FP32 and the FP64 twin; fused (value) and the
+    reference's plain sequence."""
     nx, ny, nz = 1024, 512, 256
     nc = nx // 2 + 1
     for prec, rs in ((D.Precision.SINGLE, 4), (D.Precision.DOUBLE, 8)):
-        plan = D.PlanR2C([nx, ny, nz], prec, D.Effort(args.effort), D.Executor.HIP)
-        a = D.DeviceArray(plan.alloc_bytes)
-        b = D.DeviceArray(plan.alloc_bytes)
-        c = D.DeviceArray(plan.alloc_bytes)
-        aux = D.DeviceArray(plan.alloc_bytes)
-        rng = np.random.default_rng(3)
-        a.upload(rng.standard_normal(nx * ny * nz).astype(np.float32 if rs == 4 else np.float64))
+        res = {}
+        for fused in (True, False):
+            D.set_config(enable_fused=fused)
+            try:
+                plan = D.PlanR2C([nx, ny, nz], prec, D.Effort(args.effort), D.Executor.HIP)
+            finally:
+                D.set_config()
+            a = D.DeviceArray(plan.alloc_bytes)
+            b = D.DeviceArray(plan.alloc_bytes)
+            c = D.DeviceArray(plan.alloc_bytes)
+            aux = D.DeviceArray(plan.alloc_bytes)
+            rng = np.random.default_rng(3)
+            a.upload(rng.standard_normal(nx * ny * nz).astype(np.float32 if rs == 4 else np.float64))
 
-        def cyc():
-            plan.execute(a, b, D.Execute.FORWARD, aux)
-            plan.execute(b, c, D.Execute.BACKWARD, aux)
-        ms = _time(D, timer, cyc, args.steps, args.warmup)
+            def cyc():
+                plan.execute(a, b, D.Execute.FORWARD, aux)
+                plan.execute(b, c, D.Execute.BACKWARD, aux)
+            res[fused] = _time(D, timer, cyc, args.steps, args.warmup)
+            del a, b, c, aux, plan
         R = nx * ny * nz * rs
         Cb = nc * ny * nz * 2 * rs
         algo = 2 * (R + 5 * Cb)
+        ms = res[True]
         _line(f"R2C fwd+bwd cycle GB/s, 1024x512x256 {'fp32' if rs == 4 else 'fp64'}", algo / (ms * 1e-3) / 1e9,
-              "GB/s", ms, args, "3D R2C 1024x512x256, plain sequence (R2C X, Y, Z with permutes)",
-              dtype="f32" if rs == 4 else "f64", algorithmic_bytes=algo)
+              "GB/s", ms, args, "3D R2C 1024x512x256: R2C X, X->Y+FFT_Y, Y->Z+FFT_Z; back Z, Y in place, "
+              "Z->X fused with C2R X", dtype="f32" if rs == 4 else "f64", algorithmic_bytes=algo,
+              plain={"value": round(algo / (res[False] * 1e-3) / 1e9, 2), "ms_per_step": round(res[False], 4),
+                     "form": "reference sequence: R2C X, permute, FFT Y, permute, FFT Z"},
+              fused_over_plain=round(res[False] / ms, 3))
 
 
 def config4(D, args, timer):

@@ -81,6 +81,18 @@ class KernelType(enum.IntEnum):
     UNPACK_BACKWARD_PIPELINED = 18
 
 
+class R2RKind(enum.IntEnum):
+    """dtfft_r2r_kind_t = FFTW kinds, include/dtfft_config.h.in:66-73"""
+    DCT_1 = 3
+    DCT_2 = 5
+    DCT_3 = 4
+    DCT_4 = 6
+    DST_1 = 7
+    DST_2 = 9
+    DST_3 = 8
+    DST_4 = 10
+
+
 class FFTType(enum.IntEnum):
     C2C = 0
     R2C = 1
@@ -158,10 +170,12 @@ def device_arch() -> str:
 
 
 def set_config(enable_z_slab: bool = False, enable_fused: bool = True, enable_kernel_autotune: bool = False,
-               measure_warmup_iters: int = 2, measure_iters: int = 5, stream: int | None = None) -> None:
-    """dtfft_set_config for this library (src/dtfft_config.F90)."""
+               measure_warmup_iters: int = 2, measure_iters: int = 5, stream: int | None = None,
+               lds_dma: int = 1) -> None:
+    """dtfft_set_config for this library (src/dtfft_config.F90).  ``lds_dma``: 0 never, 1 where
+    picked (default), 2 wherever allowed -- column-in passes through the LDS double buffer."""
     c = Config(int(enable_z_slab), int(enable_fused), int(enable_kernel_autotune), measure_warmup_iters,
-               measure_iters, stream)
+               measure_iters, stream, int(lds_dma))
     call("dtfft_hip_set_config", C.byref(c))
 
 
@@ -208,11 +222,11 @@ class FFTExecutor:
 
     def __init__(self, fft_rank: int, fft_type: FFTType, precision: Precision, idist: int, odist: int,
                  how_many: int, fft_sizes, inembed=None, onembed=None, effort: Effort = Effort.ESTIMATE,
-                 stream: int | None = None):
+                 stream: int | None = None, r2r_kinds=None):
         self.h = C.c_void_p()
         call("dtfft_hip_fft_create", int(fft_rank), int(fft_type), int(precision), int(idist), int(odist),
              int(how_many), _i32arr(fft_sizes), _i32arr(inembed or fft_sizes), _i32arr(onembed or fft_sizes),
-             None, int(effort), C.c_void_p(stream), C.byref(self.h))
+             None if r2r_kinds is None else _i32arr(r2r_kinds), int(effort), C.c_void_p(stream), C.byref(self.h))
 
     def execute(self, a, b, sign: int) -> None:
         call("dtfft_hip_fft_execute", self.h, C.c_void_p(_ptr(a)), C.c_void_p(_ptr(b)), int(sign))
@@ -318,6 +332,22 @@ class PlanC2C(_Plan):
 class PlanR2C(_Plan):
     """Real-to-complex plan (dtfft_create_plan_r2c for one rank)."""
     _create = "dtfft_hip_create_plan_r2c"
+
+
+class PlanR2R(_Plan):
+    """Real-to-real plan (dtfft_create_plan_r2r for one rank).  ``kinds`` in natural
+    Fortran order (X first), like ``dims``; None for a transpose-only plan."""
+    _create = "dtfft_hip_create_plan_r2r"
+
+    def __init__(self, dims, kinds=None, precision: Precision = Precision.DOUBLE, effort: Effort = Effort.ESTIMATE,
+                 executor: Executor = Executor.NONE):
+        self.dims = [int(d) for d in dims]
+        self.kinds = None if kinds is None else [int(k) for k in kinds]
+        self.h = C.c_void_p()
+        rev = list(reversed(self.dims))
+        kv = None if kinds is None else _i32arr(list(reversed(self.kinds)))
+        call(self._create, len(rev), _i32arr(rev), kv, int(precision), int(effort), int(executor),
+             C.byref(self.h))
 
 
 class UnpackFFT:

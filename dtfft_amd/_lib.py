@@ -30,6 +30,7 @@ class Config(C.Structure):
         ("measure_warmup_iters", C.c_int32),
         ("measure_iters", C.c_int32),
         ("stream", C.c_void_p),
+        ("lds_dma", C.c_int32),
     ]
 
 
@@ -56,6 +57,7 @@ SIGNATURES = {
     "dtfft_hip_set_config": (_i32, [C.POINTER(Config)]),
     "dtfft_hip_create_plan_c2c": (_i32, [_i8, _pi32, _i32, _i32, _i32, _pvp]),
     "dtfft_hip_create_plan_r2c": (_i32, [_i8, _pi32, _i32, _i32, _i32, _pvp]),
+    "dtfft_hip_create_plan_r2r": (_i32, [_i8, _pi32, _pi32, _i32, _i32, _i32, _pvp]),
     "dtfft_hip_execute": (_i32, [_vp, _vp, _vp, _i32, _vp]),
     "dtfft_hip_transpose": (_i32, [_vp, _vp, _vp, _i32, _vp]),
     "dtfft_hip_destroy": (_i32, [_pvp]),

@@ -423,7 +423,9 @@ fft_r2c_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
 // Z_k = 2E_k + i 2O_k, k < M; the backward M-point transform of Z gives
 // y_2j + i y_2j+1.  Input strides in complex units (in_ls = idist), output strides in
 // complex units of the real output view (out_ls = n/2).
-template <typename T, class S, int TPL, int LPB, int PAD>
+// IN_COLS: the complex input is read across lines (consecutive lines adjacent, element
+// stride in_ns) -- the transpose that precedes the C2R pass is fused into its load.
+template <typename T, class S, int TPL, int LPB, int PAD, bool IN_COLS = false>
 __global__ void __launch_bounds__(TPL * LPB)
 fft_c2r_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
                const typename tw_of<T>::type* __restrict__ tw,
@@ -435,12 +437,20 @@ fft_c2r_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
         i64 l0, ibase, obase;
         F::tile_bases(d, t, l0, ibase, obase);
         for (int e = threadIdx.x; e < LPB * M; e += F::NT) {
-            const int line = e / M, k = e % M;
+            int line, k;
+            if constexpr (IN_COLS) { line = e % LPB; k = e / LPB; }
+            else { line = e / M; k = e % M; }
             cx<T> z = mk((T)0, (T)0);
             if (l0 + line < d.nl) {
                 const i64 lb = ibase + (i64)line * d.in_ls;
-                cx<T> a = in[lb + k];
-                cx<T> b = in[lb + (M - k)];
+                cx<T> a, b;
+                if constexpr (IN_COLS) {
+                    a = in[lb + (i64)k * d.in_ns];
+                    b = in[lb + (i64)(M - k) * d.in_ns];
+                } else {
+                    a = in[lb + k];
+                    b = in[lb + (M - k)];
+                }
                 if (k == 0) { a.y = (T)0; b.y = (T)0; }  // Im X_0, Im X_M do not contribute
                 b = conj(b);
                 const cx<T> e2 = a + b;
@@ -544,4 +554,196 @@ __global__ void gen_rtwiddles_f64(cx<double>* tw, int off, int n, int cnt) {
     double s, c;
     sincospi(-2.0 * (double)k / (double)n, &s, &c);
     tw[off + k] = mk(c, s);
+}
+
+// ---------------------------------------------------------------------------
+// Column-in / row-out FFT pass (the fused transpose+FFT) with a double-buffered LDS tile
+// filled by direct global->LDS loads (global_load_lds_dwordx4), so that the loads of the
+// next tile overlap the passes and stores of this one.  A tile is LPB adjacent lines of the
+// whole N-point axis, LPB * sizeof(cx<T>) = 128 bytes per FFT row.
+//   * load layout (per buffer): 128-byte rows, 8 rows per 1 KiB group (one wave-wide load,
+//     lane l -> byte 16 l), groups 1152 bytes apart; element (n, line) at byte
+//       (n / 8) * 1152 + (n % 8) * 128 + line * sizeof(cx<T>);
+//   * the radix passes run in place in that layout with lanes across lines;
+//   * the last pass writes a line-major staging layout (odd line stride LD) into the same
+//     buffer, from which whole rows are stored with lanes along the row.
+// OUT_COLS: the output is column-major too (the FFT along columns in place); the last pass
+// then writes the load layout back and the store mirrors the load, 16 bytes per lane.
+// Barriers inside a tile synchronise LDS only (lgkmcnt), so the next tile's loads stay in
+// flight; they are waited for at the top of the next tile.  Twiddles live in LDS (no global
+// loads other than the tile loads).  The host uses this kernel only when nl % LPB == 0,
+// N % 8 == 0, in_ls == 1, out_ns == 1 (out_ls == 1 for OUT_COLS) and every row segment is
+// 16-byte aligned.
+// ---------------------------------------------------------------------------
+#ifndef DTFFT_LDS_BARRIER
+#define DTFFT_LDS_BARRIER() asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory")
+#endif
+#ifndef DTFFT_WAIT_ALL
+#define DTFFT_WAIT_ALL() __builtin_amdgcn_s_waitcnt(0)
+#endif
+#ifndef DTFFT_LOAD_LDS16
+#define DTFFT_LOAD_LDS16(gptr, lptr) \
+    __builtin_amdgcn_global_load_lds((const void*)(gptr), (__attribute__((address_space(3))) void*)(lptr), 16, 0, 0)
+#endif
+
+namespace dtfft {
+template <typename T, class S, int TPL, int LPB, int PAD, bool OUT_COLS>
+struct cols_dma {
+    typedef typename tw_of<T>::type tw_t;
+    static constexpr int ES = (int)sizeof(cx<T>);
+    static_assert(LPB * ES == 128, "a tile row is one 128-byte segment");
+    static constexpr int N = S::N;
+    static constexpr int NP = S::NP;
+    static_assert(N % 8 == 0 && NP > 0, "rows come in groups of 8");
+    static constexpr int NT = TPL * LPB;
+    static_assert(NT % 64 == 0, "whole waves");
+    static constexpr int NW = NT / 64;
+    static constexpr int GROUPB = 1024 + 128;
+    static constexpr int NG = N / 8;
+    static constexpr int LD = (N + PAD) | 1;
+    static constexpr int LOADB = NG * GROUPB;
+    static constexpr int STAGEB = LPB * LD * ES;
+    static constexpr int BUFE = (LOADB > STAGEB ? LOADB : STAGEB) / ES;
+    static constexpr int TWN = S::TW_SIZE > 0 ? S::TW_SIZE : 1;
+
+    // element index of (n, line) in the load layout
+    static __device__ __forceinline__ int a_off(int n, int line) {
+        return (n >> 3) * (GROUPB / ES) + (n & 7) * (128 / ES) + line;
+    }
+
+    static __device__ __forceinline__ void bases(const lines_desc& d, i64 t, i64& ibase, i64& obase) {
+        const i64 tl = t % d.ntl, b = t / d.ntl;
+        ibase = tl * LPB + b * d.in_bs;
+        obase = tl * LPB * d.out_ls + b * d.out_bs;
+    }
+
+    // this wave's share of the loads of the tile at ibase into buf
+    static __device__ __forceinline__ void load(const cx<T>* in, cx<T>* buf, const lines_desc& d, i64 ibase) {
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const int row = lane >> 3, chunk = lane & 7;
+        for (int q = wave; q < NG; q += NW) {
+            const char* g = (const char*)(in + ibase + (i64)(q * 8 + row) * d.in_ns) + chunk * 16;
+            DTFFT_LOAD_LDS16(g, (char*)buf + q * GROUPB);
+        }
+    }
+
+    template <int p>
+    static __device__ __forceinline__ void pass(cx<T>* buf, const tw_t* twl, bool inv) {
+        constexpr int R = S::radix(p);
+        constexpr int L = S::span(p);
+        constexpr int NB = N / R;
+        constexpr int IT = (NB + TPL - 1) / TPL;
+        constexpr bool EXACT = (IT * TPL == NB);
+        constexpr bool LAST = (p == NP - 1);
+        const int line = threadIdx.x % LPB, j = threadIdx.x / LPB;
+        cx<T> v[IT][R];
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int b = j + it * TPL;
+            if (EXACT || b < NB) {
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    cx<T> x = buf[a_off(b + r * NB, line)];
+                    if (p == 0 && inv) x = conj(x);
+                    v[it][r] = x;
+                }
+            }
+        }
+        DTFFT_LDS_BARRIER();  // every read of this pass precedes the in-place writes
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int b = j + it * TPL;
+            if (EXACT || b < NB) {
+                const int k = b % L;
+                if (L > 1) {
+#pragma unroll
+                    for (int r = 1; r < R; r++)
+                        v[it][r] = twmul(v[it][r], twl[S::tw_off(p) + k * (R - 1) + (r - 1)]);
+                }
+                bfly<T, R>::run(v[it]);
+                const int base = (b - k) * R + k;
+#pragma unroll
+                for (int s = 0; s < R; s++) {
+                    const int n = base + s * L;
+                    if (LAST && !OUT_COLS) buf[line * LD + n] = v[it][s];
+                    else buf[a_off(n, line)] = v[it][s];
+                }
+            }
+        }
+        DTFFT_LDS_BARRIER();
+    }
+
+    template <int p>
+    static __device__ __forceinline__ void passes(cx<T>* buf, const tw_t* twl, bool inv) {
+        if constexpr (p < NP) {
+            pass<p>(buf, twl, inv);
+            passes<p + 1>(buf, twl, inv);
+        }
+    }
+
+    // passes on a loaded tile, then whole rows to global memory
+    static __device__ __forceinline__ void tile(cx<T>* buf, cx<T>* out, const tw_t* twl, const lines_desc& d,
+                                                i64 t, bool inv) {
+        i64 ibase, obase;
+        bases(d, t, ibase, obase);
+        passes<0>(buf, twl, inv);
+        if constexpr (OUT_COLS) {
+            // mirror of load(): lane l of group q stores bytes 16 l .. 16 l + 15 of the group
+            const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+            const int row = lane >> 3, chunk = lane & 7;
+            constexpr int PER = 16 / ES;  // complex values per 16-byte chunk
+            for (int q = wave; q < NG; q += NW) {
+                const cx<T>* src = buf + q * (GROUPB / ES) + row * (128 / ES) + chunk * PER;
+                cx<T>* dst = out + obase + (i64)(q * 8 + row) * d.out_ns + chunk * PER;
+#pragma unroll
+                for (int e = 0; e < PER; e++) {
+                    cx<T> y = src[e];
+                    if (inv) y = conj(y);
+                    dst[e] = y;
+                }
+            }
+        } else {
+            const int line = threadIdx.x / TPL, j = threadIdx.x % TPL;
+            cx<T>* o = out + obase + (i64)line * d.out_ls;
+#pragma unroll 4
+            for (int n = j; n < N; n += TPL) {
+                cx<T> y = buf[line * LD + n];
+                if (inv) y = conj(y);
+                o[n] = y;
+            }
+        }
+    }
+};
+}  // namespace dtfft
+
+// Persistent: block b handles tiles b, b + gridDim.x, ...; the two LDS buffers are separate
+// arrays so that the compiler can tell the tile being loaded from the one being processed.
+template <typename T, class S, int TPL, int LPB, int PAD, bool OUT_COLS = false>
+__global__ void __launch_bounds__(TPL * LPB)
+fft_cols_dma_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
+                    const typename tw_of<T>::type* __restrict__ tw, dtfft::lines_desc d, int inv) {
+    typedef dtfft::cols_dma<T, S, TPL, LPB, PAD, OUT_COLS> K;
+    __shared__ cx<T> buf_a[K::BUFE];
+    __shared__ cx<T> buf_b[K::BUFE];
+    __shared__ typename tw_of<T>::type twl[K::TWN];
+    for (int i = threadIdx.x; i < S::TW_SIZE; i += K::NT) twl[i] = tw[i];
+    const bool iv = inv != 0;
+    i64 t = blockIdx.x, ib, ob;
+    if (t < d.ntiles) { K::bases(d, t, ib, ob); K::load(in, buf_a, d, ib); }
+    for (;;) {
+        DTFFT_WAIT_ALL();  // this tile's loads (and the previous tile's stores) are done
+        __syncthreads();
+        if (t >= d.ntiles) break;
+        i64 tn = t + gridDim.x;
+        if (tn < d.ntiles) { K::bases(d, tn, ib, ob); K::load(in, buf_b, d, ib); }
+        K::tile(buf_a, out, twl, d, t, iv);
+        t = tn;
+        DTFFT_WAIT_ALL();
+        __syncthreads();
+        if (t >= d.ntiles) break;
+        tn = t + gridDim.x;
+        if (tn < d.ntiles) { K::bases(d, tn, ib, ob); K::load(in, buf_a, d, ib); }
+        K::tile(buf_b, out, twl, d, t, iv);
+        t = tn;
+    }
 }

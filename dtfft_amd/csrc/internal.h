@@ -85,6 +85,7 @@ struct fft_config {
     bool dbl = false;
     int kind = 0;  // 0: C2C, 1: R2C (n = 2 * this->n real inputs), 2: C2R
     int seg = 0;   // > 0: FFT axis read in segments of `seg` elements (fused unpack)
+    bool dma = false;  // column-in / row-out through the LDS double buffer (fft_cols_dma_kernel)
     std::string expr() const;
 };
 
@@ -106,22 +107,58 @@ struct fft_stage {
     fft_config cfg;
     lines_desc d{};
     const rtc_kernel* k = nullptr;
+    const rtc_kernel* k_alt = nullptr;  // cfg.dma: the plain column kernel, for misaligned buffers
     std::shared_ptr<twiddles> tw;
     std::shared_ptr<twiddles> rtw;  // R2C / C2R untangling twiddles exp(-2 pi i k / 2M)
-    unsigned grid = 0, block = 0;
+    unsigned grid = 0, block = 0, grid_alt = 0;
     int32_t init(const fft_config& c, const lines_desc& desc);
     int32_t run(const void* in, void* out, int sign, hipStream_t s) const;
 };
+// Whether a column-in / row-out stage can use fft_cols_dma_kernel (shape, strides, LDS).
+bool dma_ok(const fft_config& c, const lines_desc& d);
 lines_desc make_lines(int64_t nl, int64_t nb, int64_t in_ls, int64_t in_bs, int64_t in_ns,
                       int64_t out_ls, int64_t out_bs, int64_t out_ns, int lpb);
 // Picks the configuration for a stage (heuristic, or timed when effort >= MEASURE).
 int32_t plan_fft_stage(int n, bool dbl, int64_t nl, int64_t nb, int64_t in_ls, int64_t in_bs,
                        int64_t in_ns, int64_t out_ls, int64_t out_bs, int64_t out_ns, bool inplace,
-                       int effort, hipStream_t stream, fft_stage* st);
+                       int effort, hipStream_t stream, fft_stage* st, int lds_dma = 1);
 // Real-to-complex (kind 1) or complex-to-real (kind 2) stage over contiguous lines of n
 // reals (n even); idist / odist as abstract_executor passes them (reals for the real side).
 int32_t plan_real_stage(int kind, int n, bool dbl, int64_t how_many, int64_t idist, int64_t odist,
                         int effort, hipStream_t stream, fft_stage* st);
+// R2R line geometry, mirrors r2r_desc in device/r2r.hip (strides in real elements).
+struct r2r_desc_h {
+    int64_t nl, nb;
+    int64_t in_ls, in_bs, in_es;
+    int64_t out_ls, out_bs, out_es;
+    int64_t n, m;
+    int64_t wdist;
+};
+// One R2R transform (DCT/DST 1-4, FFTW kinds 3..10) over nl x nb real lines of length n:
+// pre-process -> complex FFT of length m -> post-process, through a caller-owned work
+// buffer of work_bytes().
+struct r2r_stage {
+    int kind = 0;
+    int64_t n = 0, m = 0;
+    bool dbl = false;
+    int sign = -1;
+    r2r_desc_h d{};
+    fft_stage fft;
+    const rtc_kernel* pre = nullptr;
+    const rtc_kernel* post = nullptr;
+    int32_t init(int kind, int64_t n, bool dbl, int64_t nl, int64_t nb, int64_t in_ls, int64_t in_bs,
+                 int64_t in_es, int64_t out_ls, int64_t out_bs, int64_t out_es, int effort, hipStream_t stream);
+    size_t work_bytes() const;
+    int32_t run(const void* in, void* out, void* work, hipStream_t s) const;
+};
+bool r2r_kind_valid(int kind);
+int r2r_inverse_kind(int kind);
+
+// C2R along lines read across (fused transpose): nl lines adjacent in the input, batches
+// in_bs apart, complex element stride in_ns; real output rows of n values viewed as n/2
+// complex, line stride out_ls and batch stride out_bs in those complex units.
+int32_t plan_c2r_cols_stage(int n, bool dbl, int64_t nl, int64_t nb, int64_t in_bs, int64_t in_ns, int64_t out_ls,
+                            int64_t out_bs, fft_stage* st);
 int32_t get_rtwiddles(int n, bool dbl, std::shared_ptr<twiddles>* out);
 
 // ---- exact data movement (kernel plugin) ----------------------------------------

@@ -68,13 +68,15 @@ static bool aligned_for(const box_desc& d, int vec, const void* in, const void* 
            ((uintptr_t)in % (size_t)(vec * es)) == 0 && ((uintptr_t)out % (size_t)(vec * es)) == 0;
 }
 
-// Default launch shapes; the autotuner may replace them (tune_mover).
+// Default launch shapes; the autotuner may replace them (tune_mover).  Measured on MI355X
+// over 512^3-class permutes: 32x32 tiles win for every element size (8-byte elements: 5.3
+// TB/s against 4.0 for 64x64); 16-byte elements want 16 rows of threads.
 static void default_shape(int es, bool transpose, int& tile, int& rows, int& pad, int& vec) {
     pad = 1;
     vec = std::max(1, 16 / es);
     if (transpose) {
-        tile = es == 16 ? 32 : 64;
-        rows = es == 16 ? 8 : 4;
+        tile = 32;
+        rows = es == 16 ? 16 : 8;
     } else {
         tile = rows = 0;
     }

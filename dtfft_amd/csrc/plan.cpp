@@ -25,7 +25,7 @@ extern "C" int32_t dtfft_hip_kernel_destroy(void*);
 
 namespace dtfft_hip {
 
-static dtfft_hip_config_t g_config = {0, 1, 0, 2, 5, nullptr};
+static dtfft_hip_config_t g_config = {0, 1, 0, 2, 5, nullptr, 1};
 
 struct permute {
     void* h = nullptr;
@@ -50,6 +50,7 @@ struct plan_t {
     bool r2c = false, dbl = false, fused = false, z_slab = false;
     bool transpose_only = false;
     int effort = 0;
+    int lds_dma = 1;
     hipStream_t stream = nullptr;
     size_t elem_size = 8;            // dtfft_get_element_size: complex bytes, reals for R2C
     size_t cplx = 8;                 // bytes of one complex element
@@ -59,10 +60,18 @@ struct plan_t {
     permute t_xy, t_yx, t_yz, t_zy, t_xz, t_zx;
     fft_stage fx, fx_c2r, fy, fz, fy_cols;           // plain stages
     fft_stage u_xy, u_yz, u_zx, u_yx;                // fused stages (u_yx: 2D only)
+    fft_stage fy_zc, u_zx_c2r, u_yx_c2r;             // fused R2C backward (3D, 3D, 2D)
+    bool r2r = false;
+    int kind[3] = {0, 0, 0};                         // R2R kinds along X, Y, Z
+    r2r_stage rx, ry, rz, rxb, ryb, rzb;             // R2R forward / backward stages
+    void* work = nullptr;                            // complex work lines of the R2R stages
 
     plan_t() = default;
     plan_t(const plan_t&) = delete;
-    ~plan_t() { if (aux) hipFree(aux); }
+    ~plan_t() {
+        if (aux) hipFree(aux);
+        if (work) hipFree(work);
+    }
 };
 
 static int32_t ensure_aux(plan_t* p, void*& aux) {
@@ -88,7 +97,29 @@ static int32_t contig(plan_t* p, int64_t n, int64_t lines, fft_stage* st) {
 static int32_t fused(plan_t* p, int64_t n, int64_t nl, int64_t nb, int64_t in_bs, int64_t in_ns, int64_t out_ls,
                      int64_t out_bs, fft_stage* st) {
     return plan_fft_stage((int)n, p->dbl, nl, nb, 1, in_bs, in_ns, out_ls, out_bs, 1, false, p->effort, p->stream,
-                          st);
+                          st, p->lds_dma);
+}
+
+// R2R stages along X, Y (and Z), each over the rows of its pencil; the backward stages
+// use the inverse kinds (src/dtfft_utils.F90:447-471).
+static int32_t build_r2r(plan_t* p) {
+    const int64_t nx = p->nx, ny = p->ny, nz = p->nz;
+    const int E = p->effort;
+    auto rows = [&](r2r_stage& st, int kind, int64_t n, int64_t lines) {
+        return st.init(kind, n, p->dbl, lines, 1, n, 0, 1, n, 0, 1, E, p->stream);
+    };
+    TRY(rows(p->rx, p->kind[0], nx, ny * nz));
+    TRY(rows(p->ry, p->kind[1], ny, nz * nx));
+    TRY(rows(p->rxb, r2r_inverse_kind(p->kind[0]), nx, ny * nz));
+    TRY(rows(p->ryb, r2r_inverse_kind(p->kind[1]), ny, nz * nx));
+    size_t wb = std::max({p->rx.work_bytes(), p->ry.work_bytes(), p->rxb.work_bytes(), p->ryb.work_bytes()});
+    if (p->ndims == 3) {
+        TRY(rows(p->rz, p->kind[2], nz, nx * ny));
+        TRY(rows(p->rzb, r2r_inverse_kind(p->kind[2]), nz, nx * ny));
+        wb = std::max({wb, p->rz.work_bytes(), p->rzb.work_bytes()});
+    }
+    if (!compile_only()) HIP_TRY(hipMalloc(&p->work, std::max<size_t>(wb, 16)));
+    return DTFFT_HIP_SUCCESS;
 }
 
 static int32_t build(plan_t* p) {
@@ -99,6 +130,7 @@ static int32_t build(plan_t* p) {
         TRY(p->t_xy.create(DTFFT_HIP_KERNEL_PERMUTE_FORWARD, 2, nc, ny, 1, es, E));
         TRY(p->t_yx.create(DTFFT_HIP_KERNEL_PERMUTE_BACKWARD, 2, ny, nc, 1, es, E));
         if (p->transpose_only) return DTFFT_HIP_SUCCESS;
+        if (p->r2r) return build_r2r(p);
         if (p->r2c) {
             TRY(plan_real_stage(1, (int)nx, p->dbl, ny, nx, nc, E, p->stream, &p->fx));
             TRY(plan_real_stage(2, (int)nx, p->dbl, ny, nc, nx, E, p->stream, &p->fx_c2r));
@@ -108,7 +140,10 @@ static int32_t build(plan_t* p) {
         TRY(contig(p, ny, nc, &p->fy));
         if (p->fused) {
             TRY(fused(p, ny, nc, 1, 0, nc, ny, 0, &p->u_xy));   // X (nc,ny) -> Y (ny,nc)
-            TRY(fused(p, nx, ny, 1, 0, ny, nx, 0, &p->u_yx));   // Y (ny,nx) -> X (nx,ny)
+            if (p->r2c)  // Y (ny,nc) -> X real (nx,ny) + C2R along X
+                TRY(plan_c2r_cols_stage((int)nx, p->dbl, ny, 1, 0, ny, nx / 2, 0, &p->u_yx_c2r));
+            else
+                TRY(fused(p, nx, ny, 1, 0, ny, nx, 0, &p->u_yx));   // Y (ny,nx) -> X (nx,ny)
         }
         return DTFFT_HIP_SUCCESS;
     }
@@ -123,6 +158,7 @@ static int32_t build(plan_t* p) {
         TRY(p->t_yx.create(DTFFT_HIP_KERNEL_PERMUTE_BACKWARD, 3, ny, nz, nc, es, E));
     }
     if (p->transpose_only) return DTFFT_HIP_SUCCESS;
+    if (p->r2r) return build_r2r(p);
     if (p->r2c) {
         TRY(plan_real_stage(1, (int)nx, p->dbl, ny * nz, nx, nc, E, p->stream, &p->fx));
         TRY(plan_real_stage(2, (int)nx, p->dbl, ny * nz, nc, nx, E, p->stream, &p->fx_c2r));
@@ -132,7 +168,8 @@ static int32_t build(plan_t* p) {
     TRY(contig(p, nz, nc * ny, &p->fz));
     if (p->z_slab || p->fused) {
         // Y along the strided axis of the X-pencil (second half of the 2D FFT), in place
-        TRY(plan_fft_stage((int)ny, p->dbl, nc, nz, 1, nc * ny, nc, 1, nc * ny, nc, true, E, p->stream, &p->fy_cols));
+        TRY(plan_fft_stage((int)ny, p->dbl, nc, nz, 1, nc * ny, nc, 1, nc * ny, nc, true, E, p->stream, &p->fy_cols,
+                           p->lds_dma));
     }
     if (!p->z_slab) TRY(contig(p, ny, nz * nc, &p->fy));
     if (p->fused) {
@@ -141,13 +178,21 @@ static int32_t build(plan_t* p) {
         // row apart): forward X, X->Y+Y, Y->Z+Z; backward Z, Z->X+X, Y in place.
         TRY(fused(p, ny, nc, nz, nc * ny, nc, ny * nz, ny, &p->u_xy));      // X -> Y + FFT_Y
         TRY(fused(p, nz, ny, nc, ny * nz, ny, nz * nc, nz, &p->u_yz));      // Y -> Z + FFT_Z
-        TRY(fused(p, nx, nz, ny, nz * nc, nz, nc * ny, nc, &p->u_zx));      // Z -> X + FFT_X
+        if (p->r2c) {
+            // backward: Z, then Y in place on the Z-pencil (X must come last), then
+            // Z -> X fused with the C2R along X into the real X-pencil rows
+            TRY(plan_fft_stage((int)ny, p->dbl, nz * nc, 1, 1, 0, nz * nc, 1, 0, nz * nc, true, E, p->stream,
+                               &p->fy_zc, p->lds_dma));
+            TRY(plan_c2r_cols_stage((int)nx, p->dbl, nz, ny, nz * nc, nz, ny * (nx / 2), nx / 2, &p->u_zx_c2r));
+        } else {
+            TRY(fused(p, nx, nz, ny, nz * nc, nz, nc * ny, nc, &p->u_zx));  // Z -> X + FFT_X
+        }
     }
     return DTFFT_HIP_SUCCESS;
 }
 
 static int32_t create_plan(int8_t ndims, const int32_t* dims, int32_t precision, int32_t effort, int32_t executor,
-                           bool r2c, dtfft_hip_plan_t* out) {
+                           bool r2c, dtfft_hip_plan_t* out, bool r2r = false, const int32_t* kinds = nullptr) {
     if (!out) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
     *out = nullptr;
     if (ndims != 2 && ndims != 3) return DTFFT_HIP_ERROR_INVALID_N_DIMENSIONS;
@@ -169,11 +214,27 @@ static int32_t create_plan(int8_t ndims, const int32_t* dims, int32_t precision,
     p->transpose_only = executor == DTFFT_HIP_EXECUTOR_NONE;
     p->effort = g_config.enable_kernel_autotune && effort == DTFFT_HIP_ESTIMATE ? DTFFT_HIP_MEASURE : effort;
     p->stream = (hipStream_t)g_config.stream;
+    p->lds_dma = g_config.lds_dma;
     p->z_slab = ndims == 3 && g_config.enable_z_slab != 0;
-    p->fused = g_config.enable_fused != 0 && !r2c && !p->transpose_only;
+    p->fused = g_config.enable_fused != 0 && !p->transpose_only && !(r2c && (p->nx % 2 != 0 || p->nx < 4));
     p->nc = r2c ? p->nx / 2 + 1 : p->nx;
     p->cplx = p->dbl ? 16 : 8;
     p->elem_size = r2c ? p->cplx / 2 : p->cplx;
+    if (r2r) {
+        // real elements throughout; one rank, so the generic order (Z-slab gives the same
+        // Z-pencil) and no fusion
+        p->r2r = !p->transpose_only;
+        p->cplx = p->elem_size = p->dbl ? 8 : 4;
+        p->z_slab = false;
+        p->fused = false;
+        if (p->r2r) {
+            if (!kinds) return DTFFT_HIP_ERROR_MISSING_R2R_KINDS;
+            for (int i = 0; i < ndims; i++) {
+                if (!r2r_kind_valid(kinds[i])) return DTFFT_HIP_ERROR_INVALID_R2R_KINDS;
+                p->kind[i] = kinds[ndims - 1 - i];  // kinds come in the reversed order of dims
+            }
+        }
+    }
     const int64_t ncplx = p->nc * p->ny * p->nz;
     const int64_t nreal = p->nx * p->ny * p->nz;
     if (ncplx > INT32_MAX || nreal > INT32_MAX) return DTFFT_HIP_ERROR_INDEX_OVERFLOW;
@@ -258,6 +319,39 @@ static int32_t exec_plain(plan_t* p, void* in, void* out, bool fwd, void* aux, s
     return DTFFT_HIP_SUCCESS;
 }
 
+// R2R: the reference's plain sequences (execute_2d / execute_generic) with R2R stages.
+static int32_t exec_r2r(plan_t* p, void* in, void* out, bool fwd, void* aux, steps& r) {
+    hipStream_t s = p->stream;
+    void* w = p->work;
+    TRY(ensure_aux(p, aux));
+    if (p->ndims == 2) {
+        if (fwd) {
+            STEP(r, p->rx.run(in, aux, w, s));
+            STEP(r, p->t_xy.run(aux, out, s));
+            STEP(r, p->ry.run(out, out, w, s));
+        } else {
+            STEP(r, p->ryb.run(in, in, w, s));
+            STEP(r, p->t_yx.run(in, aux, s));
+            STEP(r, p->rxb.run(aux, out, w, s));
+        }
+        return DTFFT_HIP_SUCCESS;
+    }
+    if (fwd) {
+        STEP(r, p->rx.run(in, aux, w, s));
+        STEP(r, p->t_xy.run(aux, out, s));
+        STEP(r, p->ry.run(out, out, w, s));
+        STEP(r, p->t_yz.run(out, aux, s));
+        STEP(r, p->rz.run(aux, out, w, s));
+    } else {
+        STEP(r, p->rzb.run(in, aux, w, s));
+        STEP(r, p->t_zy.run(aux, in, s));
+        STEP(r, p->ryb.run(in, in, w, s));
+        STEP(r, p->t_yx.run(in, aux, s));
+        STEP(r, p->rxb.run(aux, out, w, s));
+    }
+    return DTFFT_HIP_SUCCESS;
+}
+
 static int32_t exec_fused(plan_t* p, void* in, void* out, bool fwd, void* aux, steps& r) {
     hipStream_t s = p->stream;
     const int F = DTFFT_HIP_FFT_FORWARD, B = DTFFT_HIP_FFT_BACKWARD;
@@ -268,7 +362,7 @@ static int32_t exec_fused(plan_t* p, void* in, void* out, bool fwd, void* aux, s
             STEP(r, p->u_xy.run(aux, out, F, s));
         } else {
             STEP(r, p->fy.run(in, aux, B, s));
-            STEP(r, p->u_yx.run(aux, out, B, s));
+            STEP(r, (p->r2c ? p->u_yx_c2r : p->u_yx).run(aux, out, B, s));
         }
         return DTFFT_HIP_SUCCESS;
     }
@@ -277,6 +371,10 @@ static int32_t exec_fused(plan_t* p, void* in, void* out, bool fwd, void* aux, s
         STEP(r, p->fx.run(in, out, F, s));
         STEP(r, p->u_xy.run(out, aux, F, s));
         STEP(r, p->u_yz.run(aux, out, F, s));
+    } else if (p->r2c) {
+        STEP(r, p->fz.run(in, aux, B, s));
+        STEP(r, p->fy_zc.run(aux, aux, B, s));
+        STEP(r, p->u_zx_c2r.run(aux, out, B, s));
     } else {
         STEP(r, p->fz.run(in, aux, B, s));
         STEP(r, p->u_zx.run(aux, out, B, s));
@@ -287,6 +385,7 @@ static int32_t exec_fused(plan_t* p, void* in, void* out, bool fwd, void* aux, s
 
 static int32_t exec_steps(plan_t* p, void* in, void* out, bool fwd, void* aux, steps& r) {
     if (p->transpose_only) return exec_transpose_only(p, in, out, fwd, aux, r);
+    if (p->r2r) return exec_r2r(p, in, out, fwd, aux, r);
     return p->fused ? exec_fused(p, in, out, fwd, aux, r) : exec_plain(p, in, out, fwd, aux, r);
 }
 
@@ -298,7 +397,7 @@ extern "C" {
 
 int32_t dtfft_hip_create_config(dtfft_hip_config_t* c) {
     if (!c) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
-    *c = dtfft_hip_config_t{0, 1, 0, 2, 5, nullptr};
+    *c = dtfft_hip_config_t{0, 1, 0, 2, 5, nullptr, 1};
     return DTFFT_HIP_SUCCESS;
 }
 
@@ -306,6 +405,7 @@ int32_t dtfft_hip_set_config(const dtfft_hip_config_t* c) {
     if (!c) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
     if (c->measure_warmup_iters < 0) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
     if (c->measure_iters <= 0) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    if (c->lds_dma < 0 || c->lds_dma > 2) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
     g_config = *c;
     return DTFFT_HIP_SUCCESS;
 }
@@ -318,6 +418,11 @@ int32_t dtfft_hip_create_plan_c2c(int8_t ndims, const int32_t* dims, int32_t pre
 int32_t dtfft_hip_create_plan_r2c(int8_t ndims, const int32_t* dims, int32_t precision, int32_t effort,
                                   int32_t executor, dtfft_hip_plan_t* plan) {
     return create_plan(ndims, dims, precision, effort, executor, true, plan);
+}
+
+int32_t dtfft_hip_create_plan_r2r(int8_t ndims, const int32_t* dims, const int32_t* kinds, int32_t precision,
+                                  int32_t effort, int32_t executor, dtfft_hip_plan_t* plan) {
+    return create_plan(ndims, dims, precision, effort, executor, false, plan, true, kinds);
 }
 
 int32_t dtfft_hip_execute(dtfft_hip_plan_t plan, void* in, void* out, int32_t execute_type, void* aux) {
@@ -468,6 +573,7 @@ int32_t dtfft_hip_report(dtfft_hip_plan_t plan) {
     };
     show("fft_x", p->fx); show("fft_y", p->fy); show("fft_z", p->fz); show("fft_ycol", p->fy_cols);
     show("xy+fft", p->u_xy); show("yz+fft", p->u_yz); show("zx+fft", p->u_zx); show("yx+fft", p->u_yx);
+    show("fft_yzc", p->fy_zc); show("zx+c2r", p->u_zx_c2r); show("yx+c2r", p->u_yx_c2r);
     return DTFFT_HIP_SUCCESS;
 }
 

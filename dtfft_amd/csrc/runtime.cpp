@@ -124,6 +124,7 @@ static void write_file_atomic(const std::string& path, const std::string& data) 
 static const char* source_text(const std::string& name) {
     if (name == "fft") return k_src_fft;
     if (name == "transpose") return k_src_transpose;
+    if (name == "r2r") return k_src_r2r;
     return nullptr;
 }
 

@@ -108,6 +108,15 @@ extern "C" {
 #define DTFFT_HIP_FFT_C2C 0
 #define DTFFT_HIP_FFT_R2C 1
 #define DTFFT_HIP_FFT_R2R 2
+/* R2R kinds = FFTW REDFT/RODFT codes, include/dtfft_config.h.in:66-73 */
+#define DTFFT_HIP_DCT_1 3
+#define DTFFT_HIP_DCT_2 5
+#define DTFFT_HIP_DCT_3 4
+#define DTFFT_HIP_DCT_4 6
+#define DTFFT_HIP_DST_1 7
+#define DTFFT_HIP_DST_2 9
+#define DTFFT_HIP_DST_3 8
+#define DTFFT_HIP_DST_4 10
 /* sign, src/include/_dtfft_private.h:27-28 */
 #define DTFFT_HIP_FFT_FORWARD  (-1)
 #define DTFFT_HIP_FFT_BACKWARD (+1)
@@ -156,13 +165,18 @@ int32_t dtfft_hip_kernel_destroy(void* handle);
  * with the same arguments: fft_sizes/inembed/onembed in FFTW (row-major) order,
  * unit element stride, distances idist/odist between consecutive transforms.
  * One plan serves both directions (is_inverse_copied in the reference).
- * r2r_kinds is required for FFT_R2R (dtFFT kind ids, include/dtfft_config.h.in:66-73). */
+ * r2r_kinds is required for FFT_R2R (dtFFT kind ids, include/dtfft_config.h.in:66-73), one per
+ * dimension in fft_sizes order; FFTW semantics, unnormalised.  The forward sign applies the
+ * kinds, the backward sign their inverses (src/dtfft_utils.F90:447-471).  An R2R stage runs
+ * a complex FFT of length n, 2n (type 4), 2(n-1) (DCT-1) or 2(n+1) (DST-1), which must
+ * factor into primes <= 13 (DTFFT_HIP_ERROR_UNSUPPORTED_SIZE otherwise). */
 int32_t dtfft_hip_fft_create(int8_t fft_rank, int8_t fft_type, int32_t precision, int32_t idist,
                              int32_t odist, int32_t how_many, const int32_t* fft_sizes,
                              const int32_t* inembed, const int32_t* onembed,
                              const int32_t* r2r_kinds, int32_t effort, void* stream, void** plan);
 /* Replaces abstract_executor%execute_private: sign = FFT_FORWARD (-1) or FFT_BACKWARD (+1).
- * For R2C, forward is real->complex and backward complex->real (in != out required). */
+ * For R2C, forward is real->complex and backward complex->real (in != out required).
+ * R2R may run in place. */
 int32_t dtfft_hip_fft_execute(void* plan, const void* in, void* out, int8_t sign);
 int32_t dtfft_hip_fft_destroy(void* plan);
 
@@ -180,6 +194,9 @@ typedef struct {
     int32_t measure_warmup_iters; /* autotune warm-up launches (default 2) */
     int32_t measure_iters;        /* autotune timed launches (default 5) */
     void* stream;                 /* hipStream_t, NULL = default stream */
+    int32_t lds_dma;              /* column-in FFT passes through the LDS double buffer filled by direct
+                                     global->LDS loads: 0 never, 1 where the autotuner (or the default
+                                     rule) picks it (default), 2 wherever the stage shape allows */
 } dtfft_hip_config_t;
 
 int32_t dtfft_hip_create_config(dtfft_hip_config_t* config);
@@ -190,6 +207,11 @@ int32_t dtfft_hip_create_plan_c2c(int8_t ndims, const int32_t* dims, int32_t pre
                                   int32_t executor, dtfft_hip_plan_t* plan);
 int32_t dtfft_hip_create_plan_r2c(int8_t ndims, const int32_t* dims, int32_t precision, int32_t effort,
                                   int32_t executor, dtfft_hip_plan_t* plan);
+/* dtfft_create_plan_r2r (include/dtfft.h): kinds[ndims] in the same reversed order as dims;
+ * real elements; the backward applies the inverse kinds.  executor NONE makes a
+ * transpose-only plan of real elements (kinds may then be NULL). */
+int32_t dtfft_hip_create_plan_r2r(int8_t ndims, const int32_t* dims, const int32_t* kinds, int32_t precision,
+                                  int32_t effort, int32_t executor, dtfft_hip_plan_t* plan);
 int32_t dtfft_hip_execute(dtfft_hip_plan_t plan, void* in, void* out, int32_t execute_type, void* aux);
 int32_t dtfft_hip_transpose(dtfft_hip_plan_t plan, void* in, void* out, int32_t transpose_type, void* aux);
 int32_t dtfft_hip_destroy(dtfft_hip_plan_t* plan);

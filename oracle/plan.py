@@ -116,3 +116,22 @@ def execute_generic_c2r(z: np.ndarray, nx: int) -> np.ndarray:
     a = _fft_last(a, FFT_BACKWARD)
     a = _permute(KERNEL_PERMUTE_BACKWARD, a, (ny, nz, ncx))
     return fft_c2r(a, nx)
+
+
+def execute_2d_r2c(x: np.ndarray) -> np.ndarray:
+    """2D R2C forward (execute_2d, src/dtfft_plan.F90:877-913, fft(1) an R2C executor):
+    R2C along X (n -> n/2+1), X->Y permute, C2C along Y.  Input (ny, nx) real; output the
+    Y-pencil (ncx, ny)."""
+    ny, nx = x.shape
+    a = fft_r2c(x)
+    a = _permute(KERNEL_PERMUTE_FORWARD, a, (nx // 2 + 1, ny))
+    return _fft_last(a, FFT_FORWARD)
+
+
+def execute_2d_c2r(z: np.ndarray, nx: int) -> np.ndarray:
+    """Inverse of execute_2d_r2c (unnormalised): C2C along Y, Y->X permute, C2R along X."""
+    ncx, ny = z.shape
+    a = _fft_last(z, FFT_BACKWARD)
+    a = _permute(KERNEL_PERMUTE_FORWARD, a, (ny, ncx))  # 2D backward remaps to forward
+    return fft_c2r(a, nx)
+

@@ -62,9 +62,15 @@ def test_argument_validation(D):
     assert L.dtfft_hip_kernel_create(7, 3, _i32([4, 4, 4]), 12, None, 0, 0, 0, C.byref(h)) == 607
     # more than 2^31-1 elements per process
     assert L.dtfft_hip_kernel_create(7, 3, _i32([2048, 2048, 1024]), 8, None, 0, 0, 0, C.byref(h)) == 605
-    # executor: R2R is not supported (as the cuFFT executor, error 101)
+    # executor R2R: kinds missing (DTFFT_ERROR_MISSING_R2R_KINDS) or not DCT/DST 1-4 (INVALID_R2R_KINDS)
     assert L.dtfft_hip_fft_create(1, 2, 0, 8, 8, 1, _i32([8]), _i32([8]), _i32([8]), None, 0, None,
-                                  C.byref(h)) == 101
+                                  C.byref(h)) == 11
+    assert L.dtfft_hip_fft_create(1, 2, 0, 8, 8, 1, _i32([8]), _i32([8]), _i32([8]), _i32([11]), 0, None,
+                                  C.byref(h)) == 12
+    # plans: R2R with an executor needs kinds; DST-1 of 100 needs a 202-point FFT (prime 101)
+    assert L.dtfft_hip_create_plan_r2r(3, _i32([4, 4, 4]), None, 0, 0, 5, C.byref(h)) == 11
+    assert L.dtfft_hip_create_plan_r2r(3, _i32([4, 4, 4]), _i32([5, 5, 2]), 0, 0, 5, C.byref(h)) == 12
+    assert L.dtfft_hip_create_plan_r2r(2, _i32([4, 100]), _i32([5, 7]), 1, 0, 5, C.byref(h)) == 602
     # executor: a length with a prime factor > 13
     assert L.dtfft_hip_fft_create(1, 0, 0, 17, 17, 1, _i32([17]), _i32([17]), _i32([17]), None, 0, None,
                                   C.byref(h)) == 602

@@ -80,6 +80,31 @@ def test_c2c_2d_256(fused, prec):
     assert np.abs(back - x).max() < (1e-13 if prec == P.DOUBLE else 1e-5)
 
 
+@pytest.mark.parametrize("prec", [P.SINGLE, P.DOUBLE])
+@pytest.mark.parametrize("dims", [(32, 64, 128), (64, 128, 256), (64, 32, 48)])
+def test_c2c_3d_lds_dma(dims, prec):
+    """Fused cycle with every eligible column-in pass forced onto the LDS double-buffer
+    kernel (lds_dma=2); (64, 128, 256) gives several tiles per persistent block, so the
+    next-tile loads overlap the current tile."""
+    nx, ny, nz = dims
+    rng = np.random.default_rng(nx * 7 + ny)
+    x = (rng.uniform(-1, 1, (nz, ny, nx)) + 1j * rng.uniform(-1, 1, (nz, ny, nx))).astype(cdt(prec))
+    D.set_config(enable_fused=True, lds_dma=2)
+    try:
+        plan = D.PlanC2C(dims, prec, D.Effort.ESTIMATE, D.Executor.HIP)
+    finally:
+        D.set_config()
+    a = D.DeviceArray.from_numpy(x, plan.alloc_bytes)
+    b = D.DeviceArray(plan.alloc_bytes)
+    c = D.DeviceArray(plan.alloc_bytes)
+    plan.execute(a, b, D.Execute.FORWARD)
+    z = b.download((ny, nx, nz), x.dtype).astype(np.complex128)
+    assert rel(z, OP.execute_generic(x.astype(np.complex128), True)) < TOL[prec]
+    plan.execute(b, c, D.Execute.BACKWARD)
+    back = c.download((nz, ny, nx), x.dtype).astype(np.complex128) / x.size
+    assert rel(back, x.astype(np.complex128)) < RT[prec] * 10
+
+
 def test_transpose_only_2d_bit_exact():
     nx, ny = 256, 256
     rng = np.random.default_rng(5)
@@ -126,13 +151,16 @@ def test_transpose_api_bit_exact(z_slab):
 
 
 @pytest.mark.parametrize("prec", [P.SINGLE, P.DOUBLE])
+@pytest.mark.parametrize("fused", [True, False])
 @pytest.mark.parametrize("z_slab", [False, True])
 @pytest.mark.parametrize("dims", [(16, 10, 6), (64, 32, 16), (60, 21, 10)])
-def test_r2c_3d(dims, z_slab, prec):
+def test_r2c_3d(dims, z_slab, fused, prec):
+    """3D R2C on one rank against the oracle's generic R2C sequence; fused: R2C along X,
+    X->Y+FFT_Y, Y->Z+FFT_Z forward, and Z, Y in place, Z->X fused with the C2R backward."""
     nx, ny, nz = dims
     rng = np.random.default_rng(nx + ny + nz)
     x = rng.standard_normal((nz, ny, nx)).astype(rdt(prec))
-    plan = make_plan(D.PlanR2C, dims, prec, z_slab=z_slab)
+    plan = make_plan(D.PlanR2C, dims, prec, fused, z_slab=z_slab)
     nc = nx // 2 + 1
     a = D.DeviceArray.from_numpy(x, plan.alloc_bytes)
     b = D.DeviceArray(plan.alloc_bytes)
@@ -144,6 +172,28 @@ def test_r2c_3d(dims, z_slab, prec):
     assert rel(z, want) < TOL[prec]
     plan.execute(b, c, D.Execute.BACKWARD)
     back = c.download((nz, ny, nx), rdt(prec)).astype(np.float64) / x.size
+    assert rel(back, x.astype(np.float64)) < RT[prec] * 10
+
+
+@pytest.mark.parametrize("prec", [P.SINGLE, P.DOUBLE])
+@pytest.mark.parametrize("fused", [True, False])
+@pytest.mark.parametrize("dims", [(16, 10), (256, 256), (60, 21)])
+def test_r2c_2d(dims, fused, prec):
+    """2D R2C on one rank (execute_2d order); fused: the Y->X transpose is fused into the
+    C2R along X."""
+    nx, ny = dims
+    rng = np.random.default_rng(nx * ny)
+    x = rng.standard_normal((ny, nx)).astype(rdt(prec))
+    plan = make_plan(D.PlanR2C, dims, prec, fused)
+    nc = nx // 2 + 1
+    a = D.DeviceArray.from_numpy(x, plan.alloc_bytes)
+    b = D.DeviceArray(plan.alloc_bytes)
+    c = D.DeviceArray(plan.alloc_bytes)
+    plan.execute(a, b, D.Execute.FORWARD)
+    z = b.download((nc, ny), cdt(prec)).astype(np.complex128)
+    assert rel(z, OP.execute_2d_r2c(x.astype(np.float64))) < TOL[prec]
+    plan.execute(b, c, D.Execute.BACKWARD)
+    back = c.download((ny, nx), rdt(prec)).astype(np.float64) / x.size
     assert rel(back, x.astype(np.float64)) < RT[prec] * 10
 
 
@@ -168,6 +218,36 @@ def test_transpose_cycle_512_c128_full_size():
     k.execute(b, a)
     k.execute(a, b, sync=True)
     assert b.download(raw.size, raw.dtype).tobytes() == raw.tobytes()
+
+
+def test_c2c_512_fp32_lds_dma_matches_default():
+    """512^3 FP32 fused cycle with the LDS double-buffer column kernel forced (64 tiles per
+    persistent block) against the default kernels: forward outputs within 1e-6 of each
+    other, round trip within 1e-6."""
+    n = 512
+    rng = np.random.default_rng(9)
+    x = np.empty(n ** 3, dtype=np.complex64)
+    x.real = rng.uniform(-1, 1, n ** 3)
+    x.imag = rng.uniform(-1, 1, n ** 3)
+    outs = []
+    for mode in (0, 2):
+        D.set_config(enable_fused=True, lds_dma=mode)
+        try:
+            plan = D.PlanC2C((n, n, n), P.SINGLE, D.Effort.ESTIMATE, D.Executor.HIP)
+        finally:
+            D.set_config()
+        a = D.DeviceArray.from_numpy(x)
+        b = D.DeviceArray(x.nbytes)
+        plan.execute(a, b, D.Execute.FORWARD)
+        outs.append(b.download(n ** 3, np.complex64))
+        if mode == 2:
+            c = D.DeviceArray(x.nbytes)
+            plan.execute(b, c, D.Execute.BACKWARD)
+            back = c.download(n ** 3, np.complex64).astype(np.complex128) / n ** 3
+            assert rel(back, x.astype(np.complex128)) < 1e-6
+            del c
+        del a, b, plan
+    assert rel(outs[1].astype(np.complex128), outs[0].astype(np.complex128)) < 1e-6
 
 
 def test_c2c_512_fp32_round_trip_full_size():

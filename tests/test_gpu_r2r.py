@@ -1,0 +1,126 @@
+"""GPU parity of the R2R (DCT / DST 1-4) stages against the R2R oracle (FFTW's
+REDFT/RODFT definitions, oracle/r2r.py): the executor plugin at rank 1 and 2, and the
+single-rank R2R plans (execute_generic / execute_2d order with R2R stages)."""
+import numpy as np
+import pytest
+
+import oracle.r2r as R
+
+pytestmark = pytest.mark.gpu
+
+D = pytest.importorskip("dtfft_amd")
+P = D.Precision
+TOL = {P.DOUBLE: 1e-12, P.SINGLE: 2e-6}
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def rdt(prec):
+    return np.float64 if prec == P.DOUBLE else np.float32
+
+
+def supported(kind, n):
+    """The complex work length must factor into primes <= 13."""
+    m = R.logical_size(kind, n) if kind in (R.DCT_1, R.DST_1) else (2 * n if kind in (R.DCT_4, R.DST_4) else n)
+    for p in (2, 3, 5, 7, 11, 13):
+        while m % p == 0:
+            m //= p
+    return m == 1
+
+
+@pytest.mark.parametrize("prec", [P.SINGLE, P.DOUBLE])
+@pytest.mark.parametrize("n", [2, 8, 17, 23, 64])
+@pytest.mark.parametrize("kind", R.KINDS)
+def test_executor_rank1(kind, n, prec):
+    if not supported(kind, n):
+        with pytest.raises(D.DtfftError) as e:
+            D.FFTExecutor(1, D.FFTType.R2R, prec, n, n, 3, [n], r2r_kinds=[kind])
+        assert e.value.code == 602
+        return
+    hm = 37
+    rng = np.random.default_rng(kind * 100 + n)
+    x = rng.standard_normal((hm, n)).astype(rdt(prec))
+    ex = D.FFTExecutor(1, D.FFTType.R2R, prec, n, n, hm, [n], r2r_kinds=[kind])
+    a = D.DeviceArray.from_numpy(x)
+    b = D.DeviceArray(x.nbytes)
+    ex.execute(a, b, D.FFT_FORWARD)
+    y = b.download((hm, n), x.dtype).astype(np.float64)
+    assert rel(y, R.r2r(x.astype(np.float64), kind)) < TOL[prec]
+    ex.execute(b, a, D.FFT_BACKWARD)  # inverse kind
+    back = a.download((hm, n), x.dtype).astype(np.float64) / R.logical_size(kind, n)
+    assert rel(back, x.astype(np.float64)) < TOL[prec] * 4
+
+
+@pytest.mark.parametrize("prec", [P.SINGLE, P.DOUBLE])
+@pytest.mark.parametrize("kinds", [(5, 6), (3, 9), (8, 4), (10, 7)])
+def test_executor_rank2(kinds, prec):
+    n0, n1, hm = 12, 20, 3  # FFTW order: n0 slow, n1 fast; kinds[i] along fft_sizes[i]
+    rng = np.random.default_rng(sum(kinds))
+    x = rng.standard_normal((hm, n0, n1)).astype(rdt(prec))
+    ex = D.FFTExecutor(2, D.FFTType.R2R, prec, n0 * n1, n0 * n1, hm, [n0, n1], r2r_kinds=list(kinds))
+    a = D.DeviceArray.from_numpy(x)
+    b = D.DeviceArray(x.nbytes)
+    ex.execute(a, a, D.FFT_FORWARD)  # in place
+    y = a.download((hm, n0, n1), x.dtype).astype(np.float64)
+    want = R.r2r(x.astype(np.float64), kinds[1])
+    want = np.swapaxes(R.r2r(np.swapaxes(want, 1, 2), kinds[0]), 1, 2)
+    assert rel(y, want) < TOL[prec]
+    ex.execute(a, b, D.FFT_BACKWARD)
+    scale = R.logical_size(kinds[0], n0) * R.logical_size(kinds[1], n1)
+    back = b.download((hm, n0, n1), x.dtype).astype(np.float64) / scale
+    assert rel(back, x.astype(np.float64)) < TOL[prec] * 4
+
+
+@pytest.mark.parametrize("prec", [P.SINGLE, P.DOUBLE])
+@pytest.mark.parametrize("dims,kinds", [((16, 10, 6), (5, 6, 9)), ((17, 12, 9), (3, 8, 10)),
+                                        ((32, 24, 20), (4, 7, 5))])
+def test_plan_r2r_3d(dims, kinds, prec):
+    nx, ny, nz = dims
+    rng = np.random.default_rng(nx + ny + nz)
+    x = rng.standard_normal((nz, ny, nx)).astype(rdt(prec))
+    plan = D.PlanR2R(dims, kinds, prec, D.Effort.ESTIMATE, D.Executor.HIP)
+    assert plan.element_size == x.itemsize and plan.alloc_size == x.size
+    a = D.DeviceArray.from_numpy(x)
+    b = D.DeviceArray(x.nbytes)
+    c = D.DeviceArray(x.nbytes)
+    plan.execute(a, b, D.Execute.FORWARD)
+    z = b.download((ny, nx, nz), x.dtype).astype(np.float64)
+    assert rel(z, R.execute_generic_r2r(x.astype(np.float64), kinds, True)) < TOL[prec]
+    plan.execute(b, c, D.Execute.BACKWARD)
+    scale = np.prod([R.logical_size(k, n) for k, n in zip(kinds, dims)])
+    back = c.download((nz, ny, nx), x.dtype).astype(np.float64) / scale
+    assert rel(back, x.astype(np.float64)) < TOL[prec] * 4
+
+
+@pytest.mark.parametrize("prec", [P.SINGLE, P.DOUBLE])
+def test_plan_r2r_2d(prec):
+    nx, ny = 24, 18
+    kinds = (R.DCT_2, R.DST_4)
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal((ny, nx)).astype(rdt(prec))
+    plan = D.PlanR2R((nx, ny), kinds, prec, D.Effort.ESTIMATE, D.Executor.HIP)
+    a = D.DeviceArray.from_numpy(x)
+    b = D.DeviceArray(x.nbytes)
+    c = D.DeviceArray(x.nbytes)
+    plan.execute(a, b, D.Execute.FORWARD)
+    y = b.download((nx, ny), x.dtype).astype(np.float64)
+    assert rel(y, R.execute_2d_r2r(x.astype(np.float64), kinds, True)) < TOL[prec]
+    plan.execute(b, c, D.Execute.BACKWARD)
+    scale = R.logical_size(kinds[0], nx) * R.logical_size(kinds[1], ny)
+    back = c.download((ny, nx), x.dtype).astype(np.float64) / scale
+    assert rel(back, x.astype(np.float64)) < TOL[prec] * 4
+
+
+def test_transpose_only_r2r_plan_bit_exact():
+    """Transpose-only R2R plan (no kinds): real elements moved exactly, X -> Z pencil."""
+    dims = (20, 12, 7)
+    nx, ny, nz = dims
+    x = np.random.default_rng(4).standard_normal((nz, ny, nx)).astype(np.float32)
+    plan = D.PlanR2R(dims, None, P.SINGLE, D.Effort.ESTIMATE, D.Executor.NONE)
+    a = D.DeviceArray.from_numpy(x)
+    b = D.DeviceArray(x.nbytes)
+    plan.execute(a, b, D.Execute.FORWARD)
+    z = b.download((ny, nx, nz), np.float32)
+    assert z.tobytes() == np.ascontiguousarray(np.transpose(x, (1, 2, 0))).tobytes()

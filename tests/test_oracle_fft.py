@@ -122,3 +122,16 @@ def test_plan_r2c_3d_against_rfftn():
     assert rel(z, np.transpose(GOLD["r2c3d_out"], (1, 2, 0))) < 1e-14
     back = OP.execute_generic_c2r(z, r.shape[-1]) / r.size
     assert rel(back, r) < 1e-14
+
+
+@pytest.mark.parametrize("shape", [(10, 16), (21, 60), (8, 4)])
+def test_plan_r2c_2d_against_rfft2(shape):
+    """2D R2C on one rank ends in the Y-pencil (nx/2+1, ny): the transpose of numpy's
+    rfft2 over (y, x); the unnormalised C2R inverse returns N x."""
+    rng = np.random.default_rng(sum(shape))
+    x = rng.standard_normal(shape)
+    z = OP.execute_2d_r2c(x)
+    assert z.shape == (shape[1] // 2 + 1, shape[0])
+    assert rel(z, np.fft.rfft2(x).T) < 1e-14
+    back = OP.execute_2d_c2r(z, shape[1]) / x.size
+    assert rel(back, x) < 1e-14
