@@ -173,7 +173,8 @@ def set_config(enable_z_slab: bool = False, enable_fused: bool = True, enable_ke
                measure_warmup_iters: int = 2, measure_iters: int = 5, stream: int | None = None,
                lds_dma: int = 1) -> None:
     """dtfft_set_config for this library (src/dtfft_config.F90).  ``lds_dma``: 0 never, 1 where
-    picked (default), 2 wherever allowed -- column-in passes through the LDS double buffer."""
+    picked (default), 2 the double-buffer kernel wherever allowed, 3 the single-buffer kernel
+    wherever allowed -- column-in passes through LDS tiles filled by direct loads."""
     c = Config(int(enable_z_slab), int(enable_fused), int(enable_kernel_autotune), measure_warmup_iters,
                measure_iters, stream, int(lds_dma))
     call("dtfft_hip_set_config", C.byref(c))

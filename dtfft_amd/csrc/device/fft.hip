@@ -219,7 +219,16 @@ struct fft_tile {
     typedef typename tw_of<T>::type tw_t;
     static constexpr int N = S::N;
     static constexpr int NP = S::NP;
-    static constexpr int LD = N + PAD;  // LDS line stride (complex elements)
+    // LDS layout of a line.  PAD 0 / 1: element n at n, line stride N + PAD.  PAD 2 / 3
+    // ("spread"): one spare slot after every 16 elements, n at n + n / 16, line stride
+    // N + N / 16 + (PAD & 1) made odd.  A Stockham pass writes its outputs R elements apart
+    // across lanes (pass 0: n = R j + s); unspread, 8-byte elements R = 8 apart land on every
+    // 16th bank and serialise up to 16 ways; spread, consecutive lanes shift by one bank pair.
+    static constexpr bool SPREAD = (PAD & 2) != 0;
+    static __device__ __host__ constexpr int lidx(int n) { return SPREAD ? n + (n >> 4) : n; }
+    // LDS distance of m elements (m a multiple of 16): lidx(n + m) == lidx(n) + lspan(m)
+    static __device__ __host__ constexpr int lspan(int m) { return SPREAD ? m + m / 16 : m; }
+    static constexpr int LD = SPREAD ? ((N + (N >> 4) + (PAD & 1)) | 1) : N + PAD;  // line stride
     static constexpr int LDS_ELEMS = LPB * LD;
     static constexpr int NT = TPL * LPB;
 
@@ -262,7 +271,9 @@ struct fft_tile {
                         if (lv) x = gin[ibase + (i64)line * d.in_ls + in_elem(n, d)];
                         v[it][r] = inv ? conj(x) : x;
                     } else {
-                        v[it][r] = lds[line * LD + n];
+                        // NB a multiple of 16: the spare slots advance by NB / 16 per r
+                        const int q = (NB % 16 == 0) ? lidx(b) + r * lspan(NB) : lidx(n);
+                        v[it][r] = lds[line * LD + q];
                     }
                 }
             }
@@ -289,7 +300,12 @@ struct fft_tile {
                             gout[obase + (i64)line * d.out_ls + (i64)n * d.out_ns] = inv ? conj(y) : y;
                         }
                     } else {
-                        lds[line * LD + n] = v[it][s];
+                        // L a multiple of 16: the spare slots advance by L / 16 per s; L = 1
+                        // with R dividing 16: base is a multiple of R, so base .. base + R - 1
+                        // share one 16-element group
+                        const int q = (L % 16 == 0) ? lidx(base) + s * lspan(L)
+                                      : (L == 1 && 16 % R == 0) ? lidx(base) + s : lidx(n);
+                        lds[line * LD + q] = v[it][s];
                     }
                 }
             }
@@ -309,9 +325,9 @@ struct fft_tile {
                 if (l0 + line >= d.nl) continue;
                 if (FIRST_G) {
                     cx<T> x = gin[ibase + (i64)line * d.in_ls];
-                    lds[line * LD] = inv ? conj(x) : x;
+                    lds[line * LD + lidx(0)] = inv ? conj(x) : x;
                 } else {
-                    cx<T> y = lds[line * LD];
+                    cx<T> y = lds[line * LD + lidx(0)];
                     gout[obase + (i64)line * d.out_ls] = inv ? conj(y) : y;
                 }
             }
@@ -327,7 +343,7 @@ struct fft_tile {
                     if (IN_COLS) { line = e % LPB; n = e / LPB; } else { line = e / N; n = e % N; }
                     cx<T> x = mk((T)0, (T)0);
                     if (l0 + line < d.nl) x = gin[ibase + (i64)line * d.in_ls + in_elem(n, d)];
-                    lds[line * LD + n] = inv ? conj(x) : x;
+                    lds[line * LD + lidx(n)] = inv ? conj(x) : x;
                 }
                 __syncthreads();
                 pass<p, false, LAST_G, OUT_COLS>(gin, gout, tw, lds, ibase, obase, d, l0, inv);
@@ -407,8 +423,8 @@ fft_r2c_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
         for (int e = threadIdx.x; e < LPB * (M + 1); e += F::NT) {
             const int line = e / (M + 1), k = e % (M + 1);
             if (l0 + line >= d.nl) continue;
-            const cx<T> a = lds[line * F::LD + (k == M ? 0 : k)];
-            const cx<T> b = conj(lds[line * F::LD + (k == 0 ? 0 : M - k)]);
+            const cx<T> a = lds[line * F::LD + F::lidx(k == M ? 0 : k)];
+            const cx<T> b = conj(lds[line * F::LD + F::lidx(k == 0 ? 0 : M - k)]);
             const cx<T> s = a + b, df = a - b;
             const cx<T> w = twmul(df, rtw[k]);           // W^k (Z_k - conj Z_{M-k})
             const cx<T> x = mk(half * (s.x + w.y), half * (s.y - w.x));  // (s - i w) / 2
@@ -459,7 +475,7 @@ fft_c2r_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
             }
             // backward transform = conj(forward(conj z)): store conj(z); the last pass
             // conjugates on its global write (inv = true)
-            lds[line * F::LD + k] = conj(z);
+            lds[line * F::LD + F::lidx(k)] = conj(z);
         }
         __syncthreads();
         F::template passes<0, false, true>(in, out, tw, lds, ibase, obase, d, l0, true);
@@ -581,9 +597,20 @@ __global__ void gen_rtwiddles_f64(cx<double>* tw, int off, int n, int cnt) {
 #ifndef DTFFT_WAIT_ALL
 #define DTFFT_WAIT_ALL() __builtin_amdgcn_s_waitcnt(0)
 #endif
+// The tile load is issued through inline assembly so that the compiler does not track it:
+// its own LDS-DMA bookkeeping cannot always tell the two tile buffers apart and would wait
+// for the prefetch before reading the other buffer.  The kernels wait for it explicitly
+// (DTFFT_WAIT_ALL).  M0 holds the wave-uniform LDS destination; lane l lands at M0 + 16 l.
 #ifndef DTFFT_LOAD_LDS16
-#define DTFFT_LOAD_LDS16(gptr, lptr) \
-    __builtin_amdgcn_global_load_lds((const void*)(gptr), (__attribute__((address_space(3))) void*)(lptr), 16, 0, 0)
+#define DTFFT_LOAD_LDS16(gptr, lptr)                                                              \
+    do {                                                                                          \
+        const unsigned lds_ = __builtin_amdgcn_readfirstlane(                                     \
+            (unsigned)(unsigned long long)((__attribute__((address_space(3))) void*)(lptr)));     \
+        asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off"                       \
+                     :                                                                            \
+                     : "s"(lds_), "v"((const void*)(gptr))                                        \
+                     : "memory", "m0");                                                           \
+    } while (0)
 #endif
 
 namespace dtfft {
@@ -684,9 +711,15 @@ struct cols_dma {
     // passes on a loaded tile, then whole rows to global memory
     static __device__ __forceinline__ void tile(cx<T>* buf, cx<T>* out, const tw_t* twl, const lines_desc& d,
                                                 i64 t, bool inv) {
+        passes<0>(buf, twl, inv);
+        store(buf, out, d, t, inv);
+    }
+
+    // the transformed tile (staging layout, or the load layout for OUT_COLS) to global memory
+    static __device__ __forceinline__ void store(const cx<T>* buf, cx<T>* out, const lines_desc& d, i64 t,
+                                                 bool inv) {
         i64 ibase, obase;
         bases(d, t, ibase, obase);
-        passes<0>(buf, twl, inv);
         if constexpr (OUT_COLS) {
             // mirror of load(): lane l of group q stores bytes 16 l .. 16 l + 15 of the group
             const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -728,22 +761,59 @@ fft_cols_dma_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
     __shared__ typename tw_of<T>::type twl[K::TWN];
     for (int i = threadIdx.x; i < S::TW_SIZE; i += K::NT) twl[i] = tw[i];
     const bool iv = inv != 0;
+    const i64 G = gridDim.x;
     i64 t = blockIdx.x, ib, ob;
     if (t < d.ntiles) { K::bases(d, t, ib, ob); K::load(in, buf_a, d, ib); }
+    DTFFT_WAIT_ALL();
+    __syncthreads();  // first tile and twiddles in LDS
+    i64 tn = t + G;
+    if (tn < d.ntiles) { K::bases(d, tn, ib, ob); K::load(in, buf_b, d, ib); }
+    // Three stages in flight: the passes of tile t run while the next tile's loads and the
+    // previous tile's stores are outstanding.  The one wait per tile (for everything issued,
+    // since loads and stores share one counter) comes after the passes, when both were issued
+    // a whole pass sequence earlier; the freed buffer then takes the tile after next.
     for (;;) {
-        DTFFT_WAIT_ALL();  // this tile's loads (and the previous tile's stores) are done
-        __syncthreads();
         if (t >= d.ntiles) break;
-        i64 tn = t + gridDim.x;
-        if (tn < d.ntiles) { K::bases(d, tn, ib, ob); K::load(in, buf_b, d, ib); }
-        K::tile(buf_a, out, twl, d, t, iv);
+        K::template passes<0>(buf_a, twl, iv);
+        DTFFT_WAIT_ALL();
+        __syncthreads();  // tile tn is in buf_b
+        K::store(buf_a, out, d, t, iv);
+        DTFFT_LDS_BARRIER();  // every wave has read its part of buf_a
+        i64 tnn = tn + G;
+        if (tnn < d.ntiles) { K::bases(d, tnn, ib, ob); K::load(in, buf_a, d, ib); }
         t = tn;
+        tn = tnn;
+        if (t >= d.ntiles) break;
+        K::template passes<0>(buf_b, twl, iv);
+        DTFFT_WAIT_ALL();
+        __syncthreads();  // tile tn is in buf_a
+        K::store(buf_b, out, d, t, iv);
+        DTFFT_LDS_BARRIER();
+        tnn = tn + G;
+        if (tnn < d.ntiles) { K::bases(d, tnn, ib, ob); K::load(in, buf_b, d, ib); }
+        t = tn;
+        tn = tnn;
+    }
+}
+
+// Single-buffer variant: one tile buffer per block (about half the LDS), so that two blocks
+// share a CU and overlap each other's loads with their passes and stores.
+template <typename T, class S, int TPL, int LPB, int PAD, bool OUT_COLS = false>
+__global__ void __launch_bounds__(TPL * LPB)
+fft_cols_dma1_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
+                     const typename tw_of<T>::type* __restrict__ tw, dtfft::lines_desc d, int inv) {
+    typedef dtfft::cols_dma<T, S, TPL, LPB, PAD, OUT_COLS> K;
+    __shared__ cx<T> buf[K::BUFE];
+    __shared__ typename tw_of<T>::type twl[K::TWN];
+    for (int i = threadIdx.x; i < S::TW_SIZE; i += K::NT) twl[i] = tw[i];
+    const bool iv = inv != 0;
+    for (i64 t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
+        i64 ib, ob;
+        K::bases(d, t, ib, ob);
+        K::load(in, buf, d, ib);
         DTFFT_WAIT_ALL();
         __syncthreads();
-        if (t >= d.ntiles) break;
-        tn = t + gridDim.x;
-        if (tn < d.ntiles) { K::bases(d, tn, ib, ob); K::load(in, buf_a, d, ib); }
-        K::tile(buf_b, out, twl, d, t, iv);
-        t = tn;
+        K::tile(buf, out, twl, d, t, iv);
+        __syncthreads();  // the staged rows are read before the next tile's loads land
     }
 }

@@ -42,8 +42,8 @@ bool factorize(int n, std::vector<int>& radices) {
 
 std::string fft_config::expr() const {
     if (dma) {
-        std::string s = std::string("fft_cols_dma_kernel<") + (dbl ? "double" : "float") + ", dtfft::sched<" +
-                        std::to_string(n);
+        std::string s = std::string(dma == 2 ? "fft_cols_dma1_kernel<" : "fft_cols_dma_kernel<") +
+                        (dbl ? "double" : "float") + ", dtfft::sched<" + std::to_string(n);
         for (int r : radices) s += ", " + std::to_string(r);
         return s + ">, " + std::to_string(tpl) + ", " + std::to_string(lpb) + ", " + std::to_string(pad) +
                (out_cols ? ", true>" : ">");
@@ -72,7 +72,7 @@ static size_t lds_dma(const fft_config& c) {
     const size_t stage = (size_t)c.lpb * (size_t)((c.n + c.pad) | 1) * es;
     size_t tw = 0, L = 1;
     for (int r : c.radices) { tw += L * (size_t)(r - 1); L *= (size_t)r; }
-    return 2 * std::max(load, stage) + std::max<size_t>(1, tw) * 16;
+    return (c.dma == 2 ? 1 : 2) * std::max(load, stage) + std::max<size_t>(1, tw) * 16;
 }
 
 bool dma_ok(const fft_config& c, const lines_desc& d) {
@@ -84,10 +84,16 @@ bool dma_ok(const fft_config& c, const lines_desc& d) {
            (d.in_ns * es) % 16 == 0 && (d.in_bs * es) % 16 == 0 && lds_dma(c) <= kMaxLds;
 }
 
+// LDS line stride of fft_tile (device/fft.hip): n + pad, or with pad 2 / 3 one spare slot per
+// 16 elements plus (pad & 1), made odd.
+static size_t lds_line(int n, int pad) {
+    return pad >= 2 ? (size_t)((n + (n >> 4) + (pad & 1)) | 1) : (size_t)(n + pad);
+}
+
 static bool valid(const fft_config& c) {
     const int threads = c.tpl * c.lpb;
     if (threads < 1 || threads > 1024) return false;
-    if ((size_t)c.lpb * (c.n + c.pad) * elem_bytes(c.dbl) > kMaxLds) return false;
+    if ((size_t)c.lpb * lds_line(c.n, c.pad) * elem_bytes(c.dbl) > kMaxLds) return false;
     return true;
 }
 
@@ -120,7 +126,7 @@ fft_config default_fft_config(int n, bool dbl, bool in_cols, bool out_cols, bool
         const int target = c.tpl >= 64 ? 256 : 128;
         c.lpb = 1;
         while (c.lpb * c.tpl < target) c.lpb *= 2;
-        while ((size_t)c.lpb * n * elem_bytes(dbl) > 64 * 1024 && c.lpb > 1) c.lpb /= 2;
+        while ((size_t)c.lpb * lds_line(n, c.pad) * elem_bytes(dbl) > 64 * 1024 && c.lpb > 1) c.lpb /= 2;
         while (c.tpl * c.lpb > 1024 && c.tpl > 1) c.tpl = (c.tpl + 1) / 2;
     }
     return c;
@@ -136,7 +142,7 @@ std::vector<fft_config> fft_candidates(int n, bool dbl, bool in_cols, bool out_c
     else lpbs = {1, 2, 4, 8, 16, 32, 64};
     for (int t : tpls)
         for (int l : lpbs)
-            for (int p : {0, 1}) {
+            for (int p : {0, 1, 2, 3}) {
                 fft_config c = base;
                 c.tpl = t; c.lpb = l; c.pad = p;
                 if (!valid(c)) continue;
@@ -244,11 +250,17 @@ int32_t fft_stage::init(const fft_config& c, const lines_desc& desc) {
             return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
         }
         fft_config alt = c;
-        alt.dma = false;
+        alt.dma = 0;
         TRY(rtc_get("fft", alt.expr(), &k_alt));
         grid_alt = grid;
-        // persistent: one block per CU (the two tile buffers take most of the LDS)
-        grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(d.ntiles, std::max(1, device().num_cus)));
+        // persistent: as many blocks as are resident (one per CU with two tile buffers)
+        int per_cu = c.dma == 2 ? 2 : 1;
+        if (!compile_only() && k && k->fn &&
+            hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k->fn, (int)block, 0) != hipSuccess)
+            per_cu = 1;
+        per_cu = std::max(1, per_cu);
+        grid = (unsigned)std::max<int64_t>(
+            1, std::min<int64_t>(d.ntiles, (int64_t)per_cu * std::max(1, device().num_cus)));
     }
     return DTFFT_HIP_SUCCESS;
 }
@@ -327,7 +339,7 @@ int32_t plan_fft_stage(int n, bool dbl, int64_t nl, int64_t nb, int64_t in_ls, i
         if (std::sscanf(rec.c_str(), "%d %d %d %d", &t, &l, &p, &dm) >= 3) {
             fft_config c = cfg;
             c.tpl = t; c.lpb = l; c.pad = p;
-            c.dma = dm != 0;
+            c.dma = dm;
             if (valid(c) && (!c.dma || dma_ok(c, d))) cfg = c;
         }
     } else if (n > 1 && effort >= DTFFT_HIP_MEASURE && !compile_only()) {
@@ -342,9 +354,11 @@ int32_t plan_fft_stage(int n, bool dbl, int64_t nl, int64_t nb, int64_t in_ls, i
         auto cands = fft_candidates(n, dbl, in_cols, out_cols, inplace);
         if (effort < DTFFT_HIP_EXHAUSTIVE && cands.size() > 12) cands.resize(12);
         for (size_t i = 0, nc = cands.size(); lds_dma != 0 && i < nc; i++) {  // LDS-DMA twins
-            fft_config c = cands[i];
-            c.dma = true;
-            if (dma_ok(c, d)) cands.push_back(c);
+            for (int v : {1, 2}) {
+                fft_config c = cands[i];
+                c.dma = v;
+                if (dma_ok(c, d)) cands.push_back(c);
+            }
         }
         for (auto& c : cands) {
             fft_stage cand;
@@ -356,20 +370,20 @@ int32_t plan_fft_stage(int n, bool dbl, int64_t nl, int64_t nb, int64_t in_ls, i
         hipFree(bi);
         if (!inplace) hipFree(bo);
         char buf[64];
-        std::snprintf(buf, sizeof buf, "%d %d %d %d", cfg.tpl, cfg.lpb, cfg.pad, cfg.dma ? 1 : 0);
+        std::snprintf(buf, sizeof buf, "%d %d %d %d", cfg.tpl, cfg.lpb, cfg.pad, cfg.dma);
         tune_store(key, buf);
         tuned = true;
     }
     if (lds_dma == 0) {
-        cfg.dma = false;
-    } else if (!cfg.dma && (lds_dma == 2 || (!tuned && !dbl))) {
+        cfg.dma = 0;
+    } else if (!cfg.dma && (lds_dma >= 2 || (!tuned && !dbl))) {
         // Default for FP32 column-in passes: the LDS double buffer measured 14 % faster on
         // MI355X (512^3: 429 us per column-in / row-out pass against 498; the in-place column
         // pass, 441 against 495 with half the threads per line).  FP64 keeps the plain kernel,
         // which the autotuner preferred there.
         fft_config c = cfg;
-        c.dma = true;
-        if (c.out_cols && lds_dma != 2) c.tpl = std::max(1, c.tpl / 2);
+        c.dma = lds_dma == 3 ? 2 : 1;
+        if (c.out_cols && lds_dma < 2) c.tpl = std::max(1, c.tpl / 2);
         if (dma_ok(c, d)) cfg = c;
     }
     return st->init(cfg, d);

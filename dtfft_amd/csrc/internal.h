@@ -85,7 +85,8 @@ struct fft_config {
     bool dbl = false;
     int kind = 0;  // 0: C2C, 1: R2C (n = 2 * this->n real inputs), 2: C2R
     int seg = 0;   // > 0: FFT axis read in segments of `seg` elements (fused unpack)
-    bool dma = false;  // column-in / row-out through the LDS double buffer (fft_cols_dma_kernel)
+    int dma = 0;  // column-in passes through LDS filled by direct loads: 1 double buffer
+                  // (fft_cols_dma_kernel), 2 single buffer, two blocks per CU (fft_cols_dma1_kernel)
     std::string expr() const;
 };
 

@@ -405,7 +405,7 @@ int32_t dtfft_hip_set_config(const dtfft_hip_config_t* c) {
     if (!c) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
     if (c->measure_warmup_iters < 0) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
     if (c->measure_iters <= 0) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
-    if (c->lds_dma < 0 || c->lds_dma > 2) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    if (c->lds_dma < 0 || c->lds_dma > 3) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
     g_config = *c;
     return DTFFT_HIP_SUCCESS;
 }

@@ -194,9 +194,10 @@ typedef struct {
     int32_t measure_warmup_iters; /* autotune warm-up launches (default 2) */
     int32_t measure_iters;        /* autotune timed launches (default 5) */
     void* stream;                 /* hipStream_t, NULL = default stream */
-    int32_t lds_dma;              /* column-in FFT passes through the LDS double buffer filled by direct
-                                     global->LDS loads: 0 never, 1 where the autotuner (or the default
-                                     rule) picks it (default), 2 wherever the stage shape allows */
+    int32_t lds_dma;              /* column-in FFT passes through LDS tiles filled by direct global->LDS
+                                     loads: 0 never, 1 where the autotuner (or the default rule) picks
+                                     them (default), 2 the double-buffer kernel wherever the stage shape
+                                     allows, 3 the single-buffer kernel (two blocks per CU) likewise */
 } dtfft_hip_config_t;
 
 int32_t dtfft_hip_create_config(dtfft_hip_config_t* config);

@@ -3,6 +3,8 @@
 Bars (stated per result): FP64 plans within 1e-12 relative L2 of the oracle; FP32
 plans within 1e-6 relative L2 of the FP64 oracle evaluated on the same (FP32-rounded)
 input.  The oracle itself is pinned by tests/test_oracle_fft.py."""
+import os
+
 import numpy as np
 import pytest
 
@@ -97,3 +99,42 @@ def test_autotuned_stage_is_correct():
     ex.execute(a, b, D.FFT_FORWARD)
     got = b.download(x.shape, x.dtype).astype(np.complex128)
     assert rel(got, O.fft_c2c(x.astype(np.complex128))) < 1e-6
+
+
+@pytest.mark.parametrize("prec", [D.Precision.SINGLE, D.Precision.DOUBLE])
+def test_tuning_records_select_spread_lds_layout(tmp_path, prec):
+    """Tuning records are honoured: records written for a row stage (pad 2) and an in-place
+    column stage (pad 3) select the spread LDS layout (one spare slot per 16 elements); the
+    1D and 2D executors must still match the oracle, both directions."""
+    L = D.lib()
+    es, c = (16, "d") if prec == D.Precision.DOUBLE else (8, "f")
+    n, hm = 512, 64
+    n0, n1, hm2 = 256, 128, 2
+    recs = {
+        f"fft|{n}|{c}|000|{hm}|1|{n},0,1|{n},0,1": "32 4 2 0",
+        f"fft|{n0}|{c}|111|{n1}|{hm2}|1,{n0 * n1},{n1}|1,{n0 * n1},{n1}": "8 16 3 0",
+    }
+    (tmp_path / "tuning.txt").write_text("".join(f"{k}\t{v}\n" for k, v in recs.items()))
+    default_dir = os.environ.get("DTFFT_HIP_CACHE_DIR") or os.path.join(os.path.dirname(D._lib.LIB_PATH), "kcache")
+    assert L.dtfft_hip_set_cache_dir(str(tmp_path).encode()) == 0
+    try:
+        rng = np.random.default_rng(5)
+        cdt = np.complex128 if es == 16 else np.complex64
+        tol = 1e-12 if es == 16 else 1e-6
+        x = (rng.uniform(-1, 1, (hm, n)) + 1j * rng.uniform(-1, 1, (hm, n))).astype(cdt)
+        ex = D.FFTExecutor(1, D.FFTType.C2C, prec, n, n, hm, [n])
+        a = D.DeviceArray.from_numpy(x)
+        b = D.DeviceArray(x.nbytes)
+        ex.execute(a, b, D.FFT_FORWARD)
+        assert rel(b.download(x.shape, cdt).astype(np.complex128), O.fft_c2c(x.astype(np.complex128))) < tol
+        ex.execute(b, a, D.FFT_BACKWARD)
+        assert rel(a.download(x.shape, cdt).astype(np.complex128) / n, x.astype(np.complex128)) < tol
+        y = (rng.uniform(-1, 1, (hm2, n0, n1)) + 1j * rng.uniform(-1, 1, (hm2, n0, n1))).astype(cdt)
+        e2 = D.FFTExecutor(2, D.FFTType.C2C, prec, n0 * n1, n0 * n1, hm2, [n0, n1])
+        a2 = D.DeviceArray.from_numpy(y)
+        b2 = D.DeviceArray(y.nbytes)
+        e2.execute(a2, b2, D.FFT_FORWARD)
+        want = OP._fft_2d(y.astype(np.complex128), D.FFT_FORWARD)
+        assert rel(b2.download(y.shape, cdt).astype(np.complex128), want) < tol
+    finally:
+        L.dtfft_hip_set_cache_dir(default_dir.encode())

@@ -81,15 +81,16 @@ def test_c2c_2d_256(fused, prec):
 
 
 @pytest.mark.parametrize("prec", [P.SINGLE, P.DOUBLE])
+@pytest.mark.parametrize("mode", [2, 3])
 @pytest.mark.parametrize("dims", [(32, 64, 128), (64, 128, 256), (64, 32, 48)])
-def test_c2c_3d_lds_dma(dims, prec):
-    """Fused cycle with every eligible column-in pass forced onto the LDS double-buffer
-    kernel (lds_dma=2); (64, 128, 256) gives several tiles per persistent block, so the
-    next-tile loads overlap the current tile."""
+def test_c2c_3d_lds_dma(dims, mode, prec):
+    """Fused cycle with every eligible column-in pass forced onto the LDS-DMA kernels
+    (mode 2: double buffer, mode 3: single buffer, two blocks per CU); (64, 128, 256) gives
+    several tiles per persistent block, so the next-tile loads overlap the current tile."""
     nx, ny, nz = dims
     rng = np.random.default_rng(nx * 7 + ny)
     x = (rng.uniform(-1, 1, (nz, ny, nx)) + 1j * rng.uniform(-1, 1, (nz, ny, nx))).astype(cdt(prec))
-    D.set_config(enable_fused=True, lds_dma=2)
+    D.set_config(enable_fused=True, lds_dma=mode)
     try:
         plan = D.PlanC2C(dims, prec, D.Effort.ESTIMATE, D.Executor.HIP)
     finally:
@@ -230,7 +231,7 @@ def test_c2c_512_fp32_lds_dma_matches_default():
     x.real = rng.uniform(-1, 1, n ** 3)
     x.imag = rng.uniform(-1, 1, n ** 3)
     outs = []
-    for mode in (0, 2):
+    for mode in (0, 2, 3):
         D.set_config(enable_fused=True, lds_dma=mode)
         try:
             plan = D.PlanC2C((n, n, n), P.SINGLE, D.Effort.ESTIMATE, D.Executor.HIP)
@@ -240,14 +241,15 @@ def test_c2c_512_fp32_lds_dma_matches_default():
         b = D.DeviceArray(x.nbytes)
         plan.execute(a, b, D.Execute.FORWARD)
         outs.append(b.download(n ** 3, np.complex64))
-        if mode == 2:
+        if mode >= 2:
             c = D.DeviceArray(x.nbytes)
             plan.execute(b, c, D.Execute.BACKWARD)
             back = c.download(n ** 3, np.complex64).astype(np.complex128) / n ** 3
             assert rel(back, x.astype(np.complex128)) < 1e-6
             del c
         del a, b, plan
-    assert rel(outs[1].astype(np.complex128), outs[0].astype(np.complex128)) < 1e-6
+    for o in outs[1:]:
+        assert rel(o.astype(np.complex128), outs[0].astype(np.complex128)) < 1e-6
 
 
 def test_c2c_512_fp32_round_trip_full_size():
