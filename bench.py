@@ -230,8 +230,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": TRAFFIC_FILE if traffic else None,
-                     "kernel": "fused transpose+FFT pass, column-in / row-out (fft_cols_dma_kernel: LDS "
-                               "double buffer filled by global_load_lds_dwordx4)",
+                     "kernel": "fused transpose+FFT pass, column-in / row-out (fft_cols_dma1_kernel: one "
+                               "LDS tile buffer filled by global_load_lds_dwordx4, one tile per block)",
                      "algorithmic_bytes_per_launch": algo_pass},
         "cpu_baseline": cpu,
     }

@@ -253,14 +253,21 @@ int32_t fft_stage::init(const fft_config& c, const lines_desc& desc) {
         alt.dma = 0;
         TRY(rtc_get("fft", alt.expr(), &k_alt));
         grid_alt = grid;
-        // persistent: as many blocks as are resident (one per CU with two tile buffers)
-        int per_cu = c.dma == 2 ? 2 : 1;
-        if (!compile_only() && k && k->fn &&
-            hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k->fn, (int)block, 0) != hipSuccess)
-            per_cu = 1;
-        per_cu = std::max(1, per_cu);
-        grid = (unsigned)std::max<int64_t>(
-            1, std::min<int64_t>(d.ntiles, (int64_t)per_cu * std::max(1, device().num_cus)));
+        if (c.dma == 1) {
+            // persistent: as many blocks as are resident (one per CU with two tile buffers)
+            int per_cu = 1;
+            if (!compile_only() && k && k->fn &&
+                hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k->fn, (int)block, 0) != hipSuccess)
+                per_cu = 1;
+            per_cu = std::max(1, per_cu);
+            grid = (unsigned)std::max<int64_t>(
+                1, std::min<int64_t>(d.ntiles, (int64_t)per_cu * std::max(1, device().num_cus)));
+        }
+        // dma == 2: one tile per block (the kernel strides by the grid beyond 2^22 tiles).
+        // Blocks start in tile order, so the blocks resident together gather neighbouring
+        // 128-byte pieces of the same rows; on MI355X this ran the 512^3 column-in / row-out
+        // pass in 430 us against 446 us for the persistent double buffer, whose blocks drift
+        // apart over their 64 tiles each.
     }
     return DTFFT_HIP_SUCCESS;
 }
@@ -377,12 +384,14 @@ int32_t plan_fft_stage(int n, bool dbl, int64_t nl, int64_t nb, int64_t in_ls, i
     if (lds_dma == 0) {
         cfg.dma = 0;
     } else if (!cfg.dma && (lds_dma >= 2 || (!tuned && !dbl))) {
-        // Default for FP32 column-in passes: the LDS double buffer measured 14 % faster on
-        // MI355X (512^3: 429 us per column-in / row-out pass against 498; the in-place column
-        // pass, 441 against 495 with half the threads per line).  FP64 keeps the plain kernel,
-        // which the autotuner preferred there.
+        // Default for FP32 column-in passes: the LDS-DMA kernels measured 10-14 % faster than
+        // the plain column kernel on MI355X.  512^3 column-in / row-out pass: 431 us with one
+        // tile per block, 444 us persistent with two buffers, 498 us plain.  The in-place
+        // column pass runs best persistent with two buffers and half the threads per line:
+        // 444 us, against 475 us with one tile per block and 495 us plain.  FP64 keeps the
+        // plain kernel, which the autotuner preferred there.
         fft_config c = cfg;
-        c.dma = lds_dma == 3 ? 2 : 1;
+        c.dma = lds_dma == 2 ? 1 : lds_dma == 3 ? 2 : (c.out_cols ? 1 : 2);
         if (c.out_cols && lds_dma < 2) c.tpl = std::max(1, c.tpl / 2);
         if (dma_ok(c, d)) cfg = c;
     }

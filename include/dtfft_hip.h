@@ -197,7 +197,8 @@ typedef struct {
     int32_t lds_dma;              /* column-in FFT passes through LDS tiles filled by direct global->LDS
                                      loads: 0 never, 1 where the autotuner (or the default rule) picks
                                      them (default), 2 the double-buffer kernel wherever the stage shape
-                                     allows, 3 the single-buffer kernel (two blocks per CU) likewise */
+                                     allows, 3 the single-buffer kernel (one tile per block, two
+                                     blocks per CU) likewise */
 } dtfft_hip_config_t;
 
 int32_t dtfft_hip_create_config(dtfft_hip_config_t* config);
