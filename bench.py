@@ -304,7 +304,8 @@ def config1(D, args, timer):
 
 def config6(D, args, timer):
     """3D R2R 512^3 FP32: DCT-2 along every axis forward, DCT-3 (its inverse) backward; the
-    reference's plain sequence with R2R stages (pre-process, C2C, post-process each)."""
+    reference's plain sequence with R2R stages (one fused kernel each: pre-process, C2C and
+    post-process through LDS) and permutes between them."""
     n = 512
     N = n ** 3
     plan = D.PlanR2R([n, n, n], [5, 5, 5], D.Precision.SINGLE, D.Effort(args.effort), D.Executor.HIP)

@@ -14,8 +14,15 @@
 //   DST-4 (10): (-1)^k DCT-4 of the reversed input
 //   DCT-1 (3):  M = 2(n-1), even extension; y_k = Re E_k
 //   DST-1 (7):  M = 2(n+1), odd extension [0, x, 0, -reverse(x)]; y_k = -Im O_{k+1}
-// Twiddles e^{i pi a} are evaluated in FP64 (sincospi) for both precisions.
-#include "common.hip"
+// Twiddles e^{i pi a} are evaluated in FP64 (sincospi) for both precisions.  Every one of
+// them is t4(j) = e^{-i pi j / 4n} or its conjugate, j < 2n; the fused kernel reads them from
+// a table of that function, the three-launch path evaluates it in place.
+//
+// Lines of unit element stride take the fused kernel fft_r2r_kernel: the real lines go to
+// LDS, the pre-process fills the complex work line in LDS, the Stockham passes of fft_tile
+// run there, and the post-process writes the real outputs: one read and one write of each
+// real element.  Other strides take the three launches through a complex work buffer.
+#include "fft.hip"  // fft_tile: the Stockham passes, here run on work lines held in LDS
 
 struct r2r_desc {
     i64 nl, nb;                // lines (l, b): line start l * ls + b * bs
@@ -32,67 +39,193 @@ __device__ __forceinline__ cx<T> cis_pi(double a) {  // e^{i pi a}
     return mk((T)c, (T)s);
 }
 
+// Element m (< M) of the complex work line.  x(j) returns real input j of the line;
+// t4(j) returns e^{-i pi j / 4n}.
+template <typename T, int KIND, class XF, class TF>
+__device__ __forceinline__ cx<T> r2r_pre_elem(const XF& x, i64 n, i64 M, i64 m, const TF& t4) {
+    cx<T> z = mk((T)0, (T)0);
+    if constexpr (KIND == 5 || KIND == 9) {
+        const T sg = (KIND == 9) ? (T)-1 : (T)1;
+        z.x = m < (n + 1) / 2 ? x(2 * m) : sg * x(2 * (n - 1 - m) + 1);
+    } else if constexpr (KIND == 4 || KIND == 8) {
+        T a, bb;
+        if (KIND == 4) { a = x(m); bb = m > 0 ? x(n - m) : (T)0; }
+        else { a = x(n - 1 - m); bb = m > 0 ? x(m - 1) : (T)0; }
+        const cx<T> w = conj(t4(2 * m));                   // e^{i pi m / 2n}
+        z = mk(a * w.x + bb * w.y, a * w.y - bb * w.x);  // (a - i bb) w
+    } else if constexpr (KIND == 6 || KIND == 10) {
+        if (m < n) {
+            const T a = KIND == 6 ? x(m) : x(n - 1 - m);
+            const cx<T> w = t4(2 * m);  // e^{-i pi m / 2n}
+            z = mk(a * w.x, a * w.y);
+        }
+    } else if constexpr (KIND == 3) {
+        z.x = m < n ? x(m) : x(M - m);
+    } else if constexpr (KIND == 7) {
+        if (m >= 1 && m <= n) z.x = x(m - 1);
+        else if (m >= n + 2) z.x = -x(M - 1 - m);
+    }
+    return z;
+}
+
+// Real output k (< n) from the transformed work line W(k).
+template <typename T, int KIND, class WF, class TF>
+__device__ __forceinline__ T r2r_post_elem(const WF& W, i64 n, i64 k, const TF& t4) {
+    if constexpr (KIND == 5 || KIND == 9) {
+        const i64 kk = KIND == 9 ? n - 1 - k : k;
+        const cx<T> w = t4(2 * kk);  // e^{-i pi kk / 2n}
+        const cx<T> c = W(kk);
+        return (T)2 * (w.x * c.x - w.y * c.y);
+    } else if constexpr (KIND == 4 || KIND == 8) {
+        const i64 idx = (k % 2 == 0) ? k / 2 : n - 1 - (k - 1) / 2;
+        T y = W(idx).x;
+        if (KIND == 8 && (k & 1)) y = -y;
+        return y;
+    } else if constexpr (KIND == 6 || KIND == 10) {
+        const cx<T> w = t4(2 * k + 1);  // e^{-i pi (2k + 1) / 4n}
+        const cx<T> c = W(k);
+        T y = (T)2 * (w.x * c.x - w.y * c.y);
+        if (KIND == 10 && (k & 1)) y = -y;
+        return y;
+    } else if constexpr (KIND == 3) {
+        return W(k).x;
+    } else {
+        return -W(k + 1).y;
+    }
+}
+
 template <typename T, int KIND>
 __global__ void __launch_bounds__(256) r2r_pre(const T* __restrict__ in, cx<T>* __restrict__ work, r2r_desc d) {
     const i64 n = d.n, M = d.m, total = d.nl * d.nb * M;
-    const i64 h = (n + 1) / 2;
+    auto t4 = [&](i64 j) { return cis_pi<T>(-(double)j / (double)(4 * n)); };
     for (i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (i64)gridDim.x * blockDim.x) {
         const i64 li = e / M, m = e % M;
         const i64 l = li % d.nl, b = li / d.nl;
         const T* x = in + l * d.in_ls + b * d.in_bs;
-        cx<T> z = mk((T)0, (T)0);
-        if constexpr (KIND == 5 || KIND == 9) {
-            const T sg = (KIND == 9) ? (T)-1 : (T)1;
-            z.x = m < h ? x[(2 * m) * d.in_es] : sg * x[(2 * (n - 1 - m) + 1) * d.in_es];
-        } else if constexpr (KIND == 4 || KIND == 8) {
-            T a, bb;
-            if (KIND == 4) { a = x[m * d.in_es]; bb = m > 0 ? x[(n - m) * d.in_es] : (T)0; }
-            else { a = x[(n - 1 - m) * d.in_es]; bb = m > 0 ? x[(m - 1) * d.in_es] : (T)0; }
-            const cx<T> w = cis_pi<T>((double)m / (double)(2 * n));
-            z = mk(a * w.x + bb * w.y, a * w.y - bb * w.x);  // (a - i bb) e^{i pi m / 2n}
-        } else if constexpr (KIND == 6 || KIND == 10) {
-            if (m < n) {
-                const T a = KIND == 6 ? x[m * d.in_es] : x[(n - 1 - m) * d.in_es];
-                const cx<T> w = cis_pi<T>(-(double)m / (double)(2 * n));
-                z = mk(a * w.x, a * w.y);
-            }
-        } else if constexpr (KIND == 3) {
-            z.x = m < n ? x[m * d.in_es] : x[(M - m) * d.in_es];
-        } else if constexpr (KIND == 7) {
-            if (m >= 1 && m <= n) z.x = x[(m - 1) * d.in_es];
-            else if (m >= n + 2) z.x = -x[(M - 1 - m) * d.in_es];
-        }
-        work[li * d.wdist + m] = z;
+        auto xf = [&](i64 j) { return x[j * d.in_es]; };
+        work[li * d.wdist + m] = r2r_pre_elem<T, KIND>(xf, n, M, m, t4);
     }
 }
 
 template <typename T, int KIND>
 __global__ void __launch_bounds__(256) r2r_post(const cx<T>* __restrict__ work, T* __restrict__ out, r2r_desc d) {
     const i64 n = d.n, total = d.nl * d.nb * n;
+    auto t4 = [&](i64 j) { return cis_pi<T>(-(double)j / (double)(4 * n)); };
     for (i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (i64)gridDim.x * blockDim.x) {
         const i64 li = e / n, k = e % n;
         const i64 l = li % d.nl, b = li / d.nl;
         const cx<T>* W = work + li * d.wdist;
-        T y;
-        if constexpr (KIND == 5 || KIND == 9) {
-            const i64 kk = KIND == 9 ? n - 1 - k : k;
-            const cx<T> w = cis_pi<T>(-(double)kk / (double)(2 * n));
-            const cx<T> c = W[kk];
-            y = (T)2 * (w.x * c.x - w.y * c.y);
-        } else if constexpr (KIND == 4 || KIND == 8) {
-            const i64 idx = (k % 2 == 0) ? k / 2 : n - 1 - (k - 1) / 2;
-            y = W[idx].x;
-            if (KIND == 8 && (k & 1)) y = -y;
-        } else if constexpr (KIND == 6 || KIND == 10) {
-            const cx<T> w = cis_pi<T>(-(double)(2 * k + 1) / (double)(4 * n));
-            const cx<T> c = W[k];
-            y = (T)2 * (w.x * c.x - w.y * c.y);
-            if (KIND == 10 && (k & 1)) y = -y;
-        } else if constexpr (KIND == 3) {
-            y = W[k].x;
-        } else {
-            y = -W[k + 1].y;
+        auto wf = [&](i64 j) { return W[j]; };
+        out[l * d.out_ls + b * d.out_bs + k * d.out_es] = r2r_post_elem<T, KIND>(wf, n, k, t4);
+    }
+}
+
+// t[j] = e^{-i pi j / 4n}, j < 2n: every twiddle of the pre- and post-processing
+template <typename T>
+__global__ void r2r_table(cx<T>* t, i64 n) {
+    const i64 j = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < 2 * n) t[j] = cis_pi<T>(-(double)j / (double)(4 * n));
+}
+
+// Real length of a KIND transform whose work line has M complex points.
+template <int KIND, int M>
+struct r2r_len {
+    static constexpr int n = KIND == 3 ? M / 2 + 1 : KIND == 7 ? M / 2 - 1 : (KIND == 6 || KIND == 10) ? M / 2 : M;
+};
+
+// Fused R2R stage over lines of unit element stride, evenly spaced: line li starts at
+// li * ls (the host folds batches into the line range, so d.nb == 1).  Persistent: block b takes tiles of LPB lines b,
+// b + gridDim.x, ...  Both twiddle tables are copied to LDS once per block, and the next
+// tile's real lines are loaded into registers while the current tile is transformed, so no
+// phase of a tile waits for global memory.  The prefetch is issued on every path (line and
+// tile indices clamped; past the end it re-reads the last line), which keeps the compiler's
+// wait counts exact.  FULL: every tile has LPB lines, so the stores need no guard (a guard
+// becomes a branch, and the compiler then waits for the stores before the next prefetch).
+template <typename T, class S, int TPL, int LPB, int PAD, int KIND, bool FULL>
+__global__ void __launch_bounds__(TPL * LPB)
+fft_r2r_kernel(const T* __restrict__ in, T* __restrict__ out, const typename tw_of<T>::type* __restrict__ tw,
+               const cx<T>* __restrict__ t4tab, r2r_desc d) {
+    typedef dtfft::fft_tile<T, S, TPL, LPB, false, false, PAD> F;
+    typedef typename tw_of<T>::type tw_t;
+    constexpr int M = S::N;
+    constexpr int NX = r2r_len<KIND, M>::n;
+    constexpr bool INV = (KIND == 4 || KIND == 8);  // types 3 run the backward transform
+    constexpr int XIT = (LPB * NX + F::NT - 1) / F::NT;  // real elements per thread
+    constexpr int MIT = (LPB * M + F::NT - 1) / F::NT;   // work elements per thread
+    constexpr int TWN = S::TW_SIZE > 0 ? S::TW_SIZE : 1;
+    __shared__ cx<T> lds[F::LDS_ELEMS];
+    __shared__ T xs[LPB * NX];
+    __shared__ tw_t twl[TWN];
+    __shared__ cx<T> t4l[2 * NX];
+    const i64 lines = d.nl, ntiles = (lines + LPB - 1) / LPB;
+    dtfft::lines_desc ld{};  // only nl is read by the LDS-to-LDS passes
+    ld.nl = lines;
+    for (int i = threadIdx.x; i < S::TW_SIZE; i += F::NT) twl[i] = tw[i];
+    for (int i = threadIdx.x; i < 2 * NX; i += F::NT) t4l[i] = t4tab[i];
+    auto t4 = [&](i64 j) { return t4l[j]; };
+    // element q of this thread in tile t: line (tid + q NT) / NX, position (tid + q NT) % NX
+    T pf[XIT];
+    auto fetch = [&](i64 t) {
+        const i64 l0 = (t < ntiles ? t : ntiles - 1) * LPB;
+#pragma unroll
+        for (int q = 0; q < XIT; q++) {
+            const int e = (int)threadIdx.x + q * F::NT;
+            const int line = e / NX, j = e % NX;
+            const i64 li = l0 + line < lines ? l0 + line : lines - 1;
+            pf[q] = in[li * d.in_ls + (XIT * F::NT == LPB * NX || e < LPB * NX ? j : 0)];
         }
-        out[l * d.out_ls + b * d.out_bs + k * d.out_es] = y;
+    };
+    auto stage_lines = [&]() {  // the prefetched lines into LDS
+#pragma unroll
+        for (int q = 0; q < XIT; q++) {
+            const int e = (int)threadIdx.x + q * F::NT;
+            if (XIT * F::NT == LPB * NX || e < LPB * NX) xs[e] = pf[q];
+        }
+    };
+    i64 t = blockIdx.x;
+    fetch(t);
+    stage_lines();
+    while (t < ntiles) {
+        const i64 l0 = t * LPB;
+        fetch(t + gridDim.x);
+        __syncthreads();  // the tile's lines (and, the first time, the twiddles) are in LDS
+#pragma unroll
+        for (int q = 0; q < MIT; q++) {
+            const int e = (int)threadIdx.x + q * F::NT;
+            if (MIT * F::NT == LPB * M || e < LPB * M) {
+                const int line = e / M, m = e % M;
+                const T* x = xs + line * NX;
+                auto xf = [&](i64 j) { return x[j]; };
+                const cx<T> z = r2r_pre_elem<T, KIND>(xf, (i64)NX, (i64)M, (i64)m, t4);
+                // backward transform = conj(forward(conj z))
+                lds[line * F::LD + F::lidx(m)] = INV ? conj(z) : z;
+            }
+        }
+        __syncthreads();
+        F::template passes<0, false, false>(nullptr, nullptr, twl, lds, 0, 0, ld, l0, false);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < XIT; q++) {
+            const int e = (int)threadIdx.x + q * F::NT;
+            if (XIT * F::NT == LPB * NX || e < LPB * NX) {
+                const int line = e / NX, k = e % NX;
+                const i64 li = l0 + line;
+                if (FULL || li < lines) {
+                    const cx<T>* W = lds + line * F::LD;
+                    auto wf = [&](i64 j) {
+                        const cx<T> c = W[F::lidx((int)j)];
+                        return INV ? conj(c) : c;
+                    };
+                    out[li * d.out_ls + k] = r2r_post_elem<T, KIND>(wf, (i64)NX, (i64)k, t4);
+                }
+            }
+        }
+        t += gridDim.x;
+        if (t >= ntiles) break;
+        // Only this path reaches here, with the tile's stores issued after the prefetch, so
+        // the wait for the prefetch lets those stores stay in flight.  The real lines were
+        // last read by the pre-process, two barriers ago; the work lines that the
+        // post-process reads are next written after the coming barrier.
+        stage_lines();
     }
 }

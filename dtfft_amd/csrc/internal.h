@@ -136,8 +136,9 @@ struct r2r_desc_h {
     int64_t wdist;
 };
 // One R2R transform (DCT/DST 1-4, FFTW kinds 3..10) over nl x nb real lines of length n:
-// pre-process -> complex FFT of length m -> post-process, through a caller-owned work
-// buffer of work_bytes().
+// pre-process -> complex FFT of length m -> post-process.  Lines of unit element stride
+// with the same layout in and out take one fused launch (fft_r2r_kernel); otherwise three
+// launches through a caller-owned work buffer of work_bytes().
 struct r2r_stage {
     int kind = 0;
     int64_t n = 0, m = 0;
@@ -147,6 +148,11 @@ struct r2r_stage {
     fft_stage fft;
     const rtc_kernel* pre = nullptr;
     const rtc_kernel* post = nullptr;
+    bool fused = false;
+    const rtc_kernel* fk = nullptr;
+    std::shared_ptr<twiddles> ftw;  // twiddles of the length-m FFT
+    std::shared_ptr<twiddles> t4;   // e^{-i pi j / 4n}, j < 2n
+    unsigned fgrid = 0, fblock = 0;
     int32_t init(int kind, int64_t n, bool dbl, int64_t nl, int64_t nb, int64_t in_ls, int64_t in_bs,
                  int64_t in_es, int64_t out_ls, int64_t out_bs, int64_t out_es, int effort, hipStream_t stream);
     size_t work_bytes() const;
@@ -154,6 +160,7 @@ struct r2r_stage {
 };
 bool r2r_kind_valid(int kind);
 int r2r_inverse_kind(int kind);
+int32_t get_r2r_table(int64_t n, bool dbl, std::shared_ptr<twiddles>* out);
 
 // C2R along lines read across (fused transpose): nl lines adjacent in the input, batches
 // in_bs apart, complex element stride in_ns; real output rows of n values viewed as n/2

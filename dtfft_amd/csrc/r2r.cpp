@@ -1,9 +1,13 @@
 // R2R stages (DCT / DST types 1-4, FFTW REDFT / RODFT semantics): the real-to-real
 // transforms dtFFT hands to FFTW (src/interfaces/fft/fftw/dtfft_executor_fftw_m.F90:
 // 149-223) or VkFFT (src/interfaces/fft/vkfft/dtfft_executor_vkfft_m.F90:97-123).  A stage
-// is three launches: r2r_pre (real line -> complex line of length M), the Stockham C2C
-// stage over the work lines, r2r_post (complex -> n real outputs); device/r2r.hip lists
-// the per-kind maps.  The work buffer (lines x M complex) belongs to the caller.
+// is r2r_pre (real line -> complex line of length M), the Stockham C2C transform of the
+// work line, and r2r_post (complex -> n real outputs); device/r2r.hip lists the per-kind
+// maps.  Contiguous lines run all three in one kernel through LDS; other layouts run three
+// launches through a work buffer (lines x M complex) that belongs to the caller.
+#include <map>
+#include <mutex>
+
 #include "internal.h"
 
 namespace dtfft_hip {
@@ -44,14 +48,79 @@ int32_t r2r_stage::init(int kind_, int64_t n_, bool dbl_, int64_t nl, int64_t nb
     m = work_length(kind, n);
     d = r2r_desc_h{nl, nb, in_ls, in_bs, in_es, out_ls, out_bs, out_es, n, m, m};
     const std::string t = dbl ? "double" : "float";
-    TRY(rtc_get("r2r", "r2r_pre<" + t + ", " + std::to_string(kind) + ">", &pre));
-    TRY(rtc_get("r2r", "r2r_post<" + t + ", " + std::to_string(kind) + ">", &post));
     // types 3 (DCT-3, DST-3) run the backward transform
     sign = (kind == 4 || kind == 8) ? DTFFT_HIP_FFT_BACKWARD : DTFFT_HIP_FFT_FORWARD;
+    std::vector<int> rad;
+    if (!factorize((int)m, rad)) {
+        set_last_error("R2R kind " + std::to_string(kind) + " of length " + std::to_string(n) +
+                       " needs a complex FFT of length " + std::to_string(m) + ", which has a prime factor > 13");
+        return DTFFT_HIP_ERROR_UNSUPPORTED_SIZE;
+    }
+    // One kernel for unit-stride, evenly spaced lines whose output overwrites nothing but the
+    // line itself: a block reads its lines whole before writing them, so in place is safe.
+    // Batches that continue the line spacing fold into one range of lines.
+    const bool even = nb == 1 || (in_bs == nl * in_ls && out_bs == nl * out_ls);
+    if (in_es == 1 && out_es == 1 && in_ls == out_ls && in_bs == out_bs && even && m > 1) {
+        fft_config c = default_fft_config((int)m, dbl, false, false, false);
+        const size_t ec = dbl ? 16 : 8, er = dbl ? 8 : 4;
+        // work lines, real lines, FFT twiddles (m - 1 of 16 bytes), table (2n complex)
+        const size_t lds = (size_t)c.lpb * (size_t)(c.n + c.pad) * ec + (size_t)c.lpb * n * er +
+                           (size_t)std::max<int64_t>(1, m - 1) * 16 + (size_t)(2 * n) * ec;
+        if (lds <= 160 * 1024) {
+            d = r2r_desc_h{nl * nb, 1, in_ls, nl * nb * in_ls, 1, out_ls, nl * nb * out_ls, 1, n, m, m};
+            std::string e = "fft_r2r_kernel<" + t + ", dtfft::sched<" + std::to_string(m);
+            for (int r : c.radices) e += ", " + std::to_string(r);
+            e += ">, " + std::to_string(c.tpl) + ", " + std::to_string(c.lpb) + ", " + std::to_string(c.pad) + ", " +
+                 std::to_string(kind) + ((nl * nb) % c.lpb == 0 ? ", true>" : ", false>");
+            TRY(rtc_get("r2r", e, &fk));
+            TRY(get_twiddles((int)m, c.radices, dbl, &ftw));
+            TRY(get_r2r_table(n, dbl, &t4));
+            const int64_t tiles = (nl * nb + c.lpb - 1) / c.lpb;
+            fblock = (unsigned)(c.tpl * c.lpb);
+            // persistent: as many blocks as are resident
+            int per_cu = 1;
+            if (!compile_only() && fk && fk->fn &&
+                hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fk->fn, (int)fblock, 0) != hipSuccess)
+                per_cu = 1;
+            per_cu = std::max(1, per_cu);
+            fgrid = (unsigned)std::max<int64_t>(
+                1, std::min<int64_t>(tiles, (int64_t)per_cu * std::max(1, device().num_cus)));
+            fused = true;
+            return DTFFT_HIP_SUCCESS;
+        }
+    }
+    TRY(rtc_get("r2r", "r2r_pre<" + t + ", " + std::to_string(kind) + ">", &pre));
+    TRY(rtc_get("r2r", "r2r_post<" + t + ", " + std::to_string(kind) + ">", &post));
     return plan_fft_stage((int)m, dbl, nl * nb, 1, m, 0, 1, m, 0, 1, true, effort, stream, &fft);
 }
 
-size_t r2r_stage::work_bytes() const { return (size_t)(d.nl * d.nb * d.wdist) * (dbl ? 16 : 8); }
+size_t r2r_stage::work_bytes() const { return fused ? 0 : (size_t)(d.nl * d.nb * d.wdist) * (dbl ? 16 : 8); }
+
+// e^{-i pi j / 4n} for j < 2n, generated on the device in FP64; one table per (n, precision)
+// for the life of the process.
+static std::mutex g_t4_mu;
+static std::map<std::pair<int64_t, bool>, std::shared_ptr<twiddles>> g_t4;
+
+int32_t get_r2r_table(int64_t n, bool dbl, std::shared_ptr<twiddles>* out) {
+    std::lock_guard<std::mutex> lk(g_t4_mu);
+    auto it = g_t4.find({n, dbl});
+    if (it != g_t4.end()) { *out = it->second; return DTFFT_HIP_SUCCESS; }
+    auto t = std::make_shared<twiddles>();
+    const rtc_kernel* k = nullptr;
+    TRY(rtc_get("r2r", std::string("r2r_table<") + (dbl ? "double" : "float") + ">", &k));
+    t->bytes = 2 * n * (dbl ? 16 : 8);
+    if (!compile_only()) {
+        HIP_TRY(hipMalloc(&t->ptr, (size_t)t->bytes));
+        void* ptr = t->ptr;
+        int64_t nn = n;
+        void* args[] = {&ptr, &nn};
+        TRY(launch(k, (unsigned)((2 * n + 255) / 256), 256, nullptr, args));
+        HIP_TRY(hipDeviceSynchronize());
+    }
+    g_t4[{n, dbl}] = t;
+    *out = t;
+    return DTFFT_HIP_SUCCESS;
+}
 
 static unsigned grid_for(int64_t items) {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>((items + 255) / 256, 1 << 20));
@@ -59,6 +128,13 @@ static unsigned grid_for(int64_t items) {
 
 int32_t r2r_stage::run(const void* in, void* out, void* work, hipStream_t s) const {
     if (d.nl * d.nb == 0) return DTFFT_HIP_SUCCESS;
+    if (fused) {
+        r2r_desc_h dd = d;
+        void* twp = ftw ? ftw->ptr : nullptr;
+        void* tp = t4 ? t4->ptr : nullptr;
+        void* args[] = {(void*)&in, &out, &twp, &tp, &dd};
+        return launch(fk, fgrid, fblock, s, args);
+    }
     if (!work) {
         set_last_error("R2R stage without a work buffer");
         return DTFFT_HIP_ERROR_INVALID_ARGUMENT;

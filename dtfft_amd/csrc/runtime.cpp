@@ -146,10 +146,11 @@ static int32_t compile(const std::string& src_name, const std::string& expr, std
         set_last_error("unknown device source " + src_name);
         return DTFFT_HIP_ERROR_RTC_COMPILE;
     }
-    const char* hdrs[] = {k_src_common};
-    const char* hdr_names[] = {"common.hip"};
+    // r2r.hip includes fft.hip for the passes of its fused kernel
+    const char* hdrs[] = {k_src_common, k_src_fft};
+    const char* hdr_names[] = {"common.hip", "fft.hip"};
     hiprtcProgram prog;
-    if (hiprtcCreateProgram(&prog, src, (src_name + ".hip").c_str(), 1, hdrs, hdr_names) != HIPRTC_SUCCESS) {
+    if (hiprtcCreateProgram(&prog, src, (src_name + ".hip").c_str(), 2, hdrs, hdr_names) != HIPRTC_SUCCESS) {
         set_last_error("hiprtcCreateProgram failed");
         return DTFFT_HIP_ERROR_RTC_COMPILE;
     }
@@ -191,6 +192,7 @@ int32_t rtc_get(const std::string& source, const std::string& expr, const rtc_ke
     const char* src = source_text(source);
     if (!src) { set_last_error("unknown source " + source); return DTFFT_HIP_ERROR_RTC_COMPILE; }
     std::string h = source + "|" + expr + "|" + std::string(src) + "|" + std::string(k_src_common);
+    if (source == "r2r") h += "|" + std::string(k_src_fft);
     for (auto& o : rtc_options()) h += "|" + o;
     char hex[32];
     std::snprintf(hex, sizeof hex, "%016llx", (unsigned long long)fnv1a(h));
