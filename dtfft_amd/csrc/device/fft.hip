@@ -230,6 +230,7 @@ struct fft_tile {
     static __device__ __host__ constexpr int lspan(int m) { return SPREAD ? m + m / 16 : m; }
     static constexpr int LD = SPREAD ? ((N + (N >> 4) + (PAD & 1)) | 1) : N + PAD;  // line stride
     static constexpr int LDS_ELEMS = LPB * LD;
+    static constexpr int TW_SIZE = S::TW_SIZE;  // twiddles of the schedule
     static constexpr int NT = TPL * LPB;
 
     // thread -> (line, j) mapping: rows (j fastest) or columns (line fastest)
@@ -514,6 +515,7 @@ template <class F0, class... Fs> struct max_lds {
     static constexpr int v = F0::LDS_ELEMS > max_lds<Fs...>::v ? F0::LDS_ELEMS : max_lds<Fs...>::v;
 };
 template <class F0> struct max_lds<F0> { static constexpr int v = F0::LDS_ELEMS; };
+template <class F0, class... Fs> struct first_of { typedef F0 type; };
 }  // namespace dtfft
 
 template <typename T, int NTMAX, class... F>
@@ -521,11 +523,31 @@ __global__ void __launch_bounds__(NTMAX)
 batched_fft_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out, const dtfft::grid_stage* __restrict__ gs,
                    const int2* __restrict__ items, int nitems, int inv) {
     __shared__ cx<T> lds[dtfft::max_lds<F...>::v];
-    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
-        const int2 w = items[it];
-        const dtfft::grid_stage& g = gs[w.x];
-        dtfft::dispatch<T, 0, F...>(g.cls, in + g.in_off, out + g.out_off, g.tw, lds, g.d, w.y, inv != 0);
+    if constexpr (sizeof...(F) == 1) {
+        // One tile shape, so one length: every grid of the launch uses the same twiddle table
+        // (the host passes one table to all of them).  The persistent block keeps it in LDS,
+        // so the passes of its tiles read no twiddle from global memory.
+        typedef typename dtfft::first_of<F...>::type F0;
+        typedef typename tw_of<T>::type tw_t;
+        __shared__ tw_t twl[F0::TW_SIZE > 0 ? F0::TW_SIZE : 1];
+        if (nitems > 0) {
+            const tw_t* tw = (const tw_t*)gs[0].tw;
+            for (int i = threadIdx.x; i < F0::TW_SIZE; i += blockDim.x) twl[i] = tw[i];
+        }
         __syncthreads();
+        for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+            const int2 w = items[it];
+            const dtfft::grid_stage& g = gs[w.x];
+            F0::run(in + g.in_off, out + g.out_off, twl, lds, g.d, w.y, inv != 0);
+            __syncthreads();
+        }
+    } else {
+        for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+            const int2 w = items[it];
+            const dtfft::grid_stage& g = gs[w.x];
+            dtfft::dispatch<T, 0, F...>(g.cls, in + g.in_off, out + g.out_off, g.tw, lds, g.d, w.y, inv != 0);
+            __syncthreads();
+        }
     }
 }
 
