@@ -392,6 +392,7 @@ int32_t plan_fft_stage(int n, bool dbl, int64_t nl, int64_t nb, int64_t in_ls, i
         // plain kernel, which the autotuner preferred there.
         fft_config c = cfg;
         c.dma = lds_dma == 2 ? 1 : lds_dma == 3 ? 2 : (c.out_cols ? 1 : 2);
+        c.lpb = 128 / elem_bytes(dbl);  // a tile row is one 128-byte piece
         if (c.out_cols && lds_dma < 2) c.tpl = std::max(1, c.tpl / 2);
         if (dma_ok(c, d)) cfg = c;
     }
