@@ -62,6 +62,11 @@ int32_t r2r_stage::init(int kind_, int64_t n_, bool dbl_, int64_t nl, int64_t nb
     const bool even = nb == 1 || (in_bs == nl * in_ls && out_bs == nl * out_ls);
     if (in_es == 1 && out_es == 1 && in_ls == out_ls && in_bs == out_bs && even && m > 1) {
         fft_config c = default_fft_config((int)m, dbl, false, false, false);
+        {
+            // one butterfly of the largest radix per thread and pass
+            const int rmax = *std::max_element(c.radices.begin(), c.radices.end());
+            if ((m / rmax) * c.lpb <= 1024) c.tpl = (int)(m / rmax);
+        }
         const size_t ec = dbl ? 16 : 8, er = dbl ? 8 : 4;
         // work lines, real lines, FFT twiddles (m - 1 of 16 bytes), table (2n complex)
         const size_t lds = (size_t)c.lpb * (size_t)(c.n + c.pad) * ec + (size_t)c.lpb * n * er +
