@@ -414,7 +414,14 @@ int32_t plan_real_stage(int kind, int n, bool dbl, int64_t how_many, int64_t idi
     }
     c = default_fft_config(m, dbl, false, false, false);
     c.kind = kind;
-    if (m == 1) { c.radices = {1}; c.tpl = 1; c.lpb = 64; }
+    if (m == 1) {
+        c.radices = {1}; c.tpl = 1; c.lpb = 64;
+    } else {
+        // one butterfly of the largest radix per thread and pass: the untangling adds
+        // registers to the row FFT, and fewer per thread keep more waves resident
+        const int rmax = *std::max_element(c.radices.begin(), c.radices.end());
+        if ((m / rmax) * c.lpb <= 1024) c.tpl = m / rmax;
+    }
     lines_desc d;
     if (kind == 1) {  // real lines (idist reals) viewed as complex; complex out (odist)
         if (idist % 2) { set_last_error("R2C idist must be even"); return DTFFT_HIP_ERROR_INVALID_ARGUMENT; }
