@@ -117,13 +117,14 @@ fft_config default_fft_config(int n, bool dbl, bool in_cols, bool out_cols, bool
         while (c.tpl * c.lpb > 1024 && c.tpl > 1) c.tpl = (c.tpl + 1) / 2;
         while (!valid(c) && c.lpb > 1) c.lpb /= 2;
     } else {
-        // Rows: small blocks (about 128 threads, 256 once a line takes 64 threads or more),
-        // several resident per CU so that one block's loads overlap another's passes; at
-        // n = 512..1024 (FP32) and 512 (FP64) two butterflies per thread and pass measured
-        // faster on MI355X (512 FP32: 5.6 TB/s against 5.2 with one).
+        // Rows: small blocks (about 128 threads in FP32, 64 in FP64, 256 once a line takes 64
+        // threads or more), several resident per CU so that one block's loads overlap
+        // another's passes; at n = 512..1024 (FP32) and 512 (FP64) two butterflies per thread
+        // and pass measured faster on MI355X (512 FP32: 5.6 TB/s against 5.2 with one; 512
+        // FP64: 731 us per pass with 2 lines per block against 763 with 4).
         c.pad = 0;
         if ((!dbl && n >= 512 && n <= 1024) || (dbl && n == 512)) c.tpl = std::max(1, c.tpl / 2);
-        const int target = c.tpl >= 64 ? 256 : 128;
+        const int target = c.tpl >= 64 ? 256 : (dbl ? 64 : 128);
         c.lpb = 1;
         while (c.lpb * c.tpl < target) c.lpb *= 2;
         while ((size_t)c.lpb * lds_line(n, c.pad) * elem_bytes(dbl) > 64 * 1024 && c.lpb > 1) c.lpb /= 2;
