@@ -154,3 +154,25 @@ def test_fused_stage_matches_three_launch_stage(kind, n, prec):
     # in place through the fused kernel
     fused.execute(a, a, D.FFT_FORWARD)
     assert a.download((hm, n), x.dtype).tobytes() == b.download((hm, n), x.dtype).tobytes()
+
+
+@pytest.mark.parametrize("prec", [P.SINGLE, P.DOUBLE])
+def test_executor_rank2_padded_batches(prec):
+    """Batches farther apart than one grid (idist = odist > n0 * n1): the fast-axis stage cannot
+    fold its batches into one evenly spaced range of lines, so it runs the three launches; the
+    slow-axis stage runs them anyway (strided lines).  The result is still the oracle's."""
+    n0, n1, hm = 10, 16, 3
+    dist = n0 * n1 + 24
+    kinds = (R.DCT_2, R.DST_4)
+    rng = np.random.default_rng(11)
+    x = np.zeros((hm, dist), dtype=rdt(prec))
+    x[:, :n0 * n1] = rng.standard_normal((hm, n0 * n1))
+    ex = D.FFTExecutor(2, D.FFTType.R2R, prec, dist, dist, hm, [n0, n1], r2r_kinds=list(kinds))
+    a = D.DeviceArray.from_numpy(x)
+    b = D.DeviceArray(x.nbytes)
+    ex.execute(a, b, D.FFT_FORWARD)
+    y = b.download((hm, dist), x.dtype)[:, :n0 * n1].reshape(hm, n0, n1).astype(np.float64)
+    g = x[:, :n0 * n1].reshape(hm, n0, n1).astype(np.float64)
+    want = R.r2r(g, kinds[1])
+    want = np.swapaxes(R.r2r(np.swapaxes(want, 1, 2), kinds[0]), 1, 2)
+    assert rel(y, want) < TOL[prec]
