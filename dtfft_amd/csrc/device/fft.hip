@@ -388,16 +388,30 @@ struct fft_tile {
 
 }  // namespace dtfft
 
-// Plain batched C2C launch: one tile per block (grid-stride over tiles).
-template <typename T, class S, int TPL, int LPB, bool IN_COLS, bool OUT_COLS, int PAD, int SEG = 0>
+// Plain batched C2C launch, grid-stride over tiles.  TWL = false: one tile per block, the
+// passes read the twiddles from global memory.  TWL = true: persistent blocks (as many as are
+// resident) copy the twiddle table into LDS once; used for long lines, whose table would
+// otherwise share the L1 cache with the tile data.
+template <typename T, class S, int TPL, int LPB, bool IN_COLS, bool OUT_COLS, int PAD, int SEG = 0, bool TWL = false>
 __global__ void __launch_bounds__(TPL * LPB)
 fft_c2c_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
                const typename tw_of<T>::type* __restrict__ tw, dtfft::lines_desc d, int inv) {
     typedef dtfft::fft_tile<T, S, TPL, LPB, IN_COLS, OUT_COLS, PAD, SEG> F;
     __shared__ cx<T> lds[F::LDS_ELEMS];
-    for (i64 t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
-        F::run(in, out, tw, lds, d, t, inv != 0);
+    if constexpr (TWL) {
+        typedef typename tw_of<T>::type tw_t;
+        __shared__ tw_t twl[F::TW_SIZE > 0 ? F::TW_SIZE : 1];
+        for (int i = threadIdx.x; i < F::TW_SIZE; i += blockDim.x) twl[i] = tw[i];
         __syncthreads();
+        for (i64 t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
+            F::run(in, out, twl, lds, d, t, inv != 0);
+            __syncthreads();
+        }
+    } else {
+        for (i64 t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
+            F::run(in, out, tw, lds, d, t, inv != 0);
+            __syncthreads();
+        }
     }
 }
 

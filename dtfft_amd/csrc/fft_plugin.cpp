@@ -55,7 +55,8 @@ std::string fft_config::expr() const {
     s += ">, " + std::to_string(tpl) + ", " + std::to_string(lpb) + ", ";
     if (kind == 0) s += std::string(in_cols ? "true" : "false") + ", " + (out_cols ? "true" : "false") + ", ";
     s += std::to_string(pad);
-    if (kind == 0 && seg > 0) s += ", " + std::to_string(seg);
+    if (kind == 0 && (seg > 0 || twl)) s += ", " + std::to_string(seg);
+    if (kind == 0 && twl) s += ", true";
     if (kind == 2 && in_cols) s += ", true";
     s += ">";
     return s;
@@ -245,6 +246,16 @@ int32_t fft_stage::init(const fft_config& c, const lines_desc& desc) {
     if (c.kind != 0) TRY(get_rtwiddles(2 * c.n, c.dbl, &rtw));
     block = (unsigned)(c.tpl * c.lpb);
     grid = (unsigned)std::min<int64_t>(d.ntiles, 1 << 22);
+    if (c.twl && c.kind == 0 && !c.dma) {
+        // persistent: as many blocks as are resident, each copying the twiddles to LDS once
+        int per_cu = 1;
+        if (!compile_only() && k && k->fn &&
+            hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k->fn, (int)block, 0) != hipSuccess)
+            per_cu = 1;
+        per_cu = std::max(1, per_cu);
+        grid = (unsigned)std::max<int64_t>(
+            1, std::min<int64_t>(d.ntiles, (int64_t)per_cu * std::max(1, device().num_cus)));
+    }
     if (c.dma) {
         if (!dma_ok(c, d)) {
             set_last_error("LDS-DMA column kernel requested for an unsuitable stage");
@@ -397,6 +408,12 @@ int32_t plan_fft_stage(int n, bool dbl, int64_t nl, int64_t nb, int64_t in_ls, i
         if (c.out_cols && lds_dma < 2) c.tpl = std::max(1, c.tpl / 2);
         if (dma_ok(c, d)) cfg = c;
     }
+    // Long FP32 rows (n >= 2048): persistent blocks with the twiddle table in LDS.  The table
+    // (n - 1 entries of 16 bytes) otherwise shares the L1 cache with the tile; on MI355X the
+    // 2048-point rows of a 2048 x 512 x 512 pencil went from 2028 to 1702 us.  At n = 512 the
+    // one-tile-per-block launch stays faster (384 against 412 us).
+    if (cfg.kind == 0 && !cfg.dma && !cfg.in_cols && !cfg.out_cols && !dbl && n >= 2048 && d.ntiles >= 2048)
+        cfg.twl = 1;
     return st->init(cfg, d);
 }
 

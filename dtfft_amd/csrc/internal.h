@@ -87,6 +87,7 @@ struct fft_config {
     int seg = 0;   // > 0: FFT axis read in segments of `seg` elements (fused unpack)
     int dma = 0;  // column-in passes through LDS filled by direct loads: 1 double buffer
                   // (fft_cols_dma_kernel), 2 single buffer, two blocks per CU (fft_cols_dma1_kernel)
+    int twl = 0;  // C2C (kind 0, dma 0): persistent blocks with the twiddle table in LDS
     std::string expr() const;
 };
 
