@@ -414,6 +414,11 @@ int32_t plan_fft_stage(int n, bool dbl, int64_t nl, int64_t nb, int64_t in_ls, i
     // one-tile-per-block launch stays faster (384 against 412 us).
     if (cfg.kind == 0 && !cfg.dma && !cfg.in_cols && !cfg.out_cols && !dbl && n >= 2048 && d.ntiles >= 2048)
         cfg.twl = 1;
+    // FP32 column-in passes that cannot take the LDS-DMA kernels (e.g. 513 lines per row, the
+    // complex X pencil of an R2C plan) run one 1024-thread block per CU anyway, so persistence
+    // costs no occupancy: with the table in LDS, 1024 x 512 x 256 R2C's X->Y pass went from
+    // 325 to 276 us.  FP64 measured no gain.
+    if (cfg.kind == 0 && !cfg.dma && cfg.in_cols && !dbl && cfg.seg == 0 && d.ntiles >= 2048) cfg.twl = 1;
     return st->init(cfg, d);
 }
 
