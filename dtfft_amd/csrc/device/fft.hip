@@ -456,14 +456,15 @@ fft_r2c_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
 // complex units of the real output view (out_ls = n/2).
 // IN_COLS: the complex input is read across lines (consecutive lines adjacent, element
 // stride in_ns) -- the transpose that precedes the C2R pass is fused into its load.
-template <typename T, class S, int TPL, int LPB, int PAD, bool IN_COLS = false>
-__global__ void __launch_bounds__(TPL * LPB)
-fft_c2r_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
-               const typename tw_of<T>::type* __restrict__ tw,
-               const typename tw_of<T>::type* __restrict__ rtw, dtfft::lines_desc d) {
-    typedef dtfft::fft_tile<T, S, TPL, LPB, false, false, PAD> F;
+// TWL: persistent blocks copy both twiddle tables into LDS once (as fft_c2c_kernel's TWL).
+namespace dtfft {
+template <typename T, class S, int TPL, int LPB, int PAD, bool IN_COLS>
+struct c2r_tiles {
+    typedef fft_tile<T, S, TPL, LPB, false, false, PAD> F;
+    typedef typename tw_of<T>::type tw_t;
+    static __device__ __forceinline__ void run(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
+                                               const tw_t* tw, const tw_t* rtw, cx<T>* lds, const lines_desc& d) {
     constexpr int M = S::N;
-    __shared__ cx<T> lds[F::LDS_ELEMS];
     for (i64 t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
         i64 l0, ibase, obase;
         F::tile_bases(d, t, l0, ibase, obase);
@@ -495,6 +496,29 @@ fft_c2r_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
         __syncthreads();
         F::template passes<0, false, true>(in, out, tw, lds, ibase, obase, d, l0, true);
         __syncthreads();
+    }
+    }
+};
+}  // namespace dtfft
+
+template <typename T, class S, int TPL, int LPB, int PAD, bool IN_COLS = false, bool TWL = false>
+__global__ void __launch_bounds__(TPL * LPB)
+fft_c2r_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
+               const typename tw_of<T>::type* __restrict__ tw,
+               const typename tw_of<T>::type* __restrict__ rtw, dtfft::lines_desc d) {
+    typedef dtfft::c2r_tiles<T, S, TPL, LPB, PAD, IN_COLS> K;
+    typedef typename tw_of<T>::type tw_t;
+    __shared__ cx<T> lds[K::F::LDS_ELEMS];
+    if constexpr (TWL) {
+        constexpr int TWN = S::TW_SIZE > 0 ? S::TW_SIZE : 1;
+        __shared__ tw_t twl[TWN];
+        __shared__ tw_t rtwl[S::N + 1];
+        for (int i = threadIdx.x; i < S::TW_SIZE; i += blockDim.x) twl[i] = tw[i];
+        for (int i = threadIdx.x; i <= S::N; i += blockDim.x) rtwl[i] = rtw[i];
+        __syncthreads();
+        K::run(in, out, twl, rtwl, lds, d);
+    } else {
+        K::run(in, out, tw, rtw, lds, d);
     }
 }
 
