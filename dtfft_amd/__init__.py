@@ -171,12 +171,14 @@ def device_arch() -> str:
 
 def set_config(enable_z_slab: bool = False, enable_fused: bool = True, enable_kernel_autotune: bool = False,
                measure_warmup_iters: int = 2, measure_iters: int = 5, stream: int | None = None,
-               lds_dma: int = 1) -> None:
+               lds_dma: int = 1, twl: int = 1) -> None:
     """dtfft_set_config for this library (src/dtfft_config.F90).  ``lds_dma``: 0 never, 1 where
     picked (default), 2 the double-buffer kernel wherever allowed, 3 the single-buffer kernel
-    wherever allowed -- column-in passes through LDS tiles filled by direct loads."""
+    wherever allowed -- column-in passes through LDS tiles filled by direct loads.  ``twl``:
+    FFT kernels with the twiddle table in LDS (persistent blocks): 0 never, 1 by the default
+    rule, 2 wherever the kernel has that form and the LDS budget allows it."""
     c = Config(int(enable_z_slab), int(enable_fused), int(enable_kernel_autotune), measure_warmup_iters,
-               measure_iters, stream, int(lds_dma))
+               measure_iters, stream, int(lds_dma), int(twl))
     call("dtfft_hip_set_config", C.byref(c))
 
 
@@ -294,6 +296,12 @@ class _Plan:
     def execute(self, inp, out, execute_type: Execute, aux=None) -> None:
         call("dtfft_hip_execute", self.h, C.c_void_p(_ptr(inp)), C.c_void_p(_ptr(out)), int(execute_type),
              C.c_void_p(_ptr(aux)))
+
+    def execute_cycle_graph(self, inp, out, in2, aux=None) -> None:
+        """Forward (inp -> out) then backward (out -> in2), captured once as a hipGraph and
+        replayed (dtfft_hip_execute_cycle_graph)."""
+        call("dtfft_hip_execute_cycle_graph", self.h, C.c_void_p(_ptr(inp)), C.c_void_p(_ptr(out)),
+             C.c_void_p(_ptr(aux)), C.c_void_p(_ptr(in2)))
 
     def execute_stage(self, inp, out, execute_type: Execute, stage: int, aux=None) -> int:
         """Run launch `stage` of the execute sequence only (profiling); returns the

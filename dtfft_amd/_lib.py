@@ -31,6 +31,7 @@ class Config(C.Structure):
         ("measure_iters", C.c_int32),
         ("stream", C.c_void_p),
         ("lds_dma", C.c_int32),
+        ("twl", C.c_int32),
     ]
 
 
@@ -61,6 +62,7 @@ SIGNATURES = {
     "dtfft_hip_execute": (_i32, [_vp, _vp, _vp, _i32, _vp]),
     "dtfft_hip_transpose": (_i32, [_vp, _vp, _vp, _i32, _vp]),
     "dtfft_hip_destroy": (_i32, [_pvp]),
+    "dtfft_hip_execute_cycle_graph": (_i32, [_vp, _vp, _vp, _vp, _vp]),
     "dtfft_hip_get_local_sizes": (_i32, [_vp, _pi32, _pi32, _pi32, _pi32, _psz]),
     "dtfft_hip_get_alloc_size": (_i32, [_vp, _psz]),
     "dtfft_hip_get_element_size": (_i32, [_vp, _psz]),

@@ -561,12 +561,17 @@ __global__ void __launch_bounds__(NTMAX)
 batched_fft_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out, const dtfft::grid_stage* __restrict__ gs,
                    const int2* __restrict__ items, int nitems, int inv) {
     __shared__ cx<T> lds[dtfft::max_lds<F...>::v];
-    if constexpr (sizeof...(F) == 1) {
+    typedef typename dtfft::first_of<F...>::type F0;
+    typedef typename tw_of<T>::type tw_t;
+    // 160 KiB of LDS per workgroup on gfx950; 16 bytes of slack for the table's alignment
+    constexpr bool TW_FITS = (long long)dtfft::max_lds<F...>::v * (long long)sizeof(cx<T>) +
+                                 (long long)(F0::TW_SIZE > 0 ? F0::TW_SIZE : 1) * (long long)sizeof(tw_t) + 16 <=
+                             160 * 1024;
+    if constexpr (sizeof...(F) == 1 && TW_FITS) {
         // One tile shape, so one length: every grid of the launch uses the same twiddle table
         // (the host passes one table to all of them).  The persistent block keeps it in LDS,
-        // so the passes of its tiles read no twiddle from global memory.
-        typedef typename dtfft::first_of<F...>::type F0;
-        typedef typename tw_of<T>::type tw_t;
+        // so the passes of its tiles read no twiddle from global memory.  (A tile that leaves
+        // no room for the table reads it from global memory, in the branch below.)
         __shared__ tw_t twl[F0::TW_SIZE > 0 ? F0::TW_SIZE : 1];
         if (nitems > 0) {
             const tw_t* tw = (const tw_t*)gs[0].tw;

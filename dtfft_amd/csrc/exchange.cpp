@@ -59,7 +59,7 @@ int32_t dtfft_hip_unpack_fft_create(int32_t ndims, const int32_t* dims, const in
     lines_desc d = make_lines(ny * nz, 1, seg, 0, 1, nx, 0, 1, c.lpb);
     d.in_seg_stride = chunk;
     // long FP32 lines keep the twiddle table in LDS (persistent blocks), as plain rows do
-    if (!dbl && nx >= 2048 && d.ntiles >= 2048) c.twl = 1;
+    apply_twl(c, d, global_config().twl);
     TRY(h->st.init(c, d));
     h->din0 = neighbor_data[3];  // din of peer 0, applied as a pointer offset
     *handle = h.release();

@@ -92,11 +92,54 @@ static size_t lds_line(int n, int pad) {
     return pad >= 2 ? (size_t)((n + (n >> 4) + (pad & 1)) | 1) : (size_t)(n + pad);
 }
 
+// Entries of the schedule's twiddle table: sum over passes of L_p * (R_p - 1).
+static size_t tw_entries(const fft_config& c) {
+    size_t tw = 0, L = 1;
+    for (int r : c.radices) { tw += L * (size_t)(r - 1); L *= (size_t)r; }
+    return tw;
+}
+
+size_t fft_lds_bytes(const fft_config& c) {
+    if (c.dma) return lds_dma(c);
+    size_t b = (size_t)c.lpb * lds_line(c.n, c.pad) * elem_bytes(c.dbl);
+    if (c.twl) {
+        // the tables are separate __shared__ arrays of 16-byte entries (FP32 hi+lo pairs or
+        // FP64 complex); 16 bytes of slack each for their alignment
+        b += std::max<size_t>(1, tw_entries(c)) * 16 + 16;
+        if (c.kind == 2) b += (size_t)(c.n + 1) * 16 + 16;  // C2R untangling table, M + 1 entries
+    }
+    return b;
+}
+
 static bool valid(const fft_config& c) {
     const int threads = c.tpl * c.lpb;
     if (threads < 1 || threads > 1024) return false;
-    if ((size_t)c.lpb * lds_line(c.n, c.pad) * elem_bytes(c.dbl) > kMaxLds) return false;
+    if (fft_lds_bytes(c) > kMaxLds) return false;
     return true;
+}
+
+void apply_twl(fft_config& c, const lines_desc& d, int mode) {
+    c.twl = 0;
+    // the LDS-DMA column kernels keep their twiddles in LDS anyway; R2C rows have no such form
+    if (mode == 0 || c.dma || c.kind == 1 || c.n < 2) return;
+    const int64_t ntiles = ((d.nl + c.lpb - 1) / c.lpb) * d.nb;
+    bool want = mode >= 2;
+    if (mode == 1) {
+        // Default rule, measured on MI355X.  Long FP32 rows (n >= 2048, plain or fused
+        // unpack+FFT): the table (n - 1 entries of 16 bytes) otherwise shares the L1 cache with
+        // the tile; the 2048-point rows of a 2048 x 512 x 512 pencil went from 2028 to 1702 us.
+        // At n = 512 the one-tile-per-block launch stays faster (384 against 412 us).  FP32
+        // column-in passes without LDS-DMA (e.g. 513 lines per row, the complex X pencil of an
+        // R2C plan) run one 1024-thread block per CU anyway, so persistence costs no occupancy:
+        // 1024 x 512 x 256 R2C's X->Y pass went from 325 to 276 us; the fused Z->X C2R of the
+        // same plan from 453 to 386 us.  FP64 measured no gain.
+        const bool rows = c.kind == 0 && !c.in_cols && !c.out_cols;
+        const bool cols = (c.kind == 0 && c.in_cols && c.seg == 0) || (c.kind == 2 && c.in_cols);
+        want = !c.dbl && ntiles >= 2048 && ((rows && c.n >= 2048) || cols);
+    }
+    if (!want) return;
+    c.twl = 1;
+    if (!valid(c)) c.twl = 0;  // tile + tables over the LDS budget: global twiddles
 }
 
 fft_config default_fft_config(int n, bool dbl, bool in_cols, bool out_cols, bool inplace) {
@@ -351,16 +394,18 @@ int32_t plan_fft_stage(int n, bool dbl, int64_t nl, int64_t nb, int64_t in_ls, i
                   (int)in_cols, (int)out_cols, (int)inplace, (long long)nl, (long long)nb, (long long)in_ls,
                   (long long)in_bs, (long long)in_ns, (long long)out_ls, (long long)out_bs, (long long)out_ns);
     const std::string key(kb);
+    const int twl_mode = global_config().twl;
     std::string rec;
     bool tuned = false;
+    int rec_twl = -1;  // twl of a tuning record (-1: the record predates the field)
     if (n > 1 && tune_lookup(key, rec)) {
         tuned = true;
-        int t, l, p, dm = 0;
-        if (std::sscanf(rec.c_str(), "%d %d %d %d", &t, &l, &p, &dm) >= 3) {
+        int t, l, p, dm = 0, tw = -1;
+        if (std::sscanf(rec.c_str(), "%d %d %d %d %d", &t, &l, &p, &dm, &tw) >= 3) {
             fft_config c = cfg;
             c.tpl = t; c.lpb = l; c.pad = p;
             c.dma = dm;
-            if (valid(c) && (!c.dma || dma_ok(c, d))) cfg = c;
+            if (valid(c) && (!c.dma || dma_ok(c, d))) { cfg = c; rec_twl = tw; }
         }
     } else if (n > 1 && effort >= DTFFT_HIP_MEASURE && !compile_only()) {
         const int64_t si = span(d, true, n), so = span(d, false, n);
@@ -380,6 +425,16 @@ int32_t plan_fft_stage(int n, bool dbl, int64_t nl, int64_t nb, int64_t in_ls, i
                 if (dma_ok(c, d)) cands.push_back(c);
             }
         }
+        // each candidate is timed in the form it would run: with the LDS twiddle table where
+        // the rule of this configuration gives it one, and at EXHAUSTIVE also in the other form
+        for (size_t i = 0, nc = cands.size(); i < nc; i++) {
+            apply_twl(cands[i], d, twl_mode);
+            if (effort >= DTFFT_HIP_EXHAUSTIVE && twl_mode != 0 && !cands[i].dma) {
+                fft_config c = cands[i];
+                apply_twl(c, d, c.twl ? 0 : 2);
+                if (c.twl != cands[i].twl) cands.push_back(c);
+            }
+        }
         for (auto& c : cands) {
             fft_stage cand;
             if (cand.init(c, d)) continue;
@@ -390,9 +445,10 @@ int32_t plan_fft_stage(int n, bool dbl, int64_t nl, int64_t nb, int64_t in_ls, i
         hipFree(bi);
         if (!inplace) hipFree(bo);
         char buf[64];
-        std::snprintf(buf, sizeof buf, "%d %d %d %d", cfg.tpl, cfg.lpb, cfg.pad, cfg.dma);
+        std::snprintf(buf, sizeof buf, "%d %d %d %d %d", cfg.tpl, cfg.lpb, cfg.pad, cfg.dma, cfg.twl);
         tune_store(key, buf);
         tuned = true;
+        rec_twl = cfg.twl;
     }
     if (lds_dma == 0) {
         cfg.dma = 0;
@@ -407,19 +463,15 @@ int32_t plan_fft_stage(int n, bool dbl, int64_t nl, int64_t nb, int64_t in_ls, i
         c.dma = lds_dma == 2 ? 1 : lds_dma == 3 ? 2 : (c.out_cols ? 1 : 2);
         c.lpb = 128 / elem_bytes(dbl);  // a tile row is one 128-byte piece
         if (c.out_cols && lds_dma < 2) c.tpl = std::max(1, c.tpl / 2);
-        if (dma_ok(c, d)) cfg = c;
+        if (dma_ok(c, d)) { cfg = c; rec_twl = -1; }
     }
-    // Long FP32 rows (n >= 2048): persistent blocks with the twiddle table in LDS.  The table
-    // (n - 1 entries of 16 bytes) otherwise shares the L1 cache with the tile; on MI355X the
-    // 2048-point rows of a 2048 x 512 x 512 pencil went from 2028 to 1702 us.  At n = 512 the
-    // one-tile-per-block launch stays faster (384 against 412 us).
-    if (cfg.kind == 0 && !cfg.dma && !cfg.in_cols && !cfg.out_cols && !dbl && n >= 2048 && d.ntiles >= 2048)
-        cfg.twl = 1;
-    // FP32 column-in passes that cannot take the LDS-DMA kernels (e.g. 513 lines per row, the
-    // complex X pencil of an R2C plan) run one 1024-thread block per CU anyway, so persistence
-    // costs no occupancy: with the table in LDS, 1024 x 512 x 256 R2C's X->Y pass went from
-    // 325 to 276 us.  FP64 measured no gain.
-    if (cfg.kind == 0 && !cfg.dma && cfg.in_cols && !dbl && cfg.seg == 0 && d.ntiles >= 2048) cfg.twl = 1;
+    // persistent blocks with the twiddle table in LDS: as the tuning record says (the form it
+    // was timed in), else by the rule of the configuration
+    if (rec_twl >= 0 && twl_mode == 1) {
+        apply_twl(cfg, d, rec_twl ? 2 : 0);
+    } else {
+        apply_twl(cfg, d, twl_mode);
+    }
     return st->init(cfg, d);
 }
 
@@ -454,6 +506,7 @@ int32_t plan_real_stage(int kind, int n, bool dbl, int64_t how_many, int64_t idi
         if (odist % 2) { set_last_error("C2R odist must be even"); return DTFFT_HIP_ERROR_INVALID_ARGUMENT; }
         d = make_lines(how_many, 1, idist, 0, 1, odist / 2, 0, 1, c.lpb);
     }
+    apply_twl(c, d, global_config().twl);  // C2R rows: only when forced (no gain measured)
     return st->init(c, d);
 }
 
@@ -472,9 +525,9 @@ int32_t plan_c2r_cols_stage(int n, bool dbl, int64_t nl, int64_t nb, int64_t in_
     c = default_fft_config(m, dbl, true, false, false);
     c.kind = 2;
     lines_desc d = make_lines(nl, nb, 1, in_bs, in_ns, out_ls, out_bs, 1, c.lpb);
-    // FP32: persistent blocks with both twiddle tables in LDS (one 1024-thread block per CU
-    // either way); at 1024 x 512 x 256 the fused Z->X C2R went from 453 to 386 us
-    if (!dbl && d.ntiles >= 2048) c.twl = 1;
+    // persistent blocks with both twiddle tables in LDS by the configuration's rule (FP32 by
+    // default: one 1024-thread block per CU either way)
+    apply_twl(c, d, global_config().twl);
     return st->init(c, d);
 }
 

@@ -45,6 +45,8 @@ struct device_info {
 };
 const device_info& device();
 bool compile_only();           // no launches / allocations (no device or forced)
+// The configuration set by dtfft_hip_set_config (process-wide, as dtfft_set_config).
+const dtfft_hip_config_t& global_config();
 
 // ---- hipRTC program cache ----------------------------------------------------------
 // A compiled kernel: module + function.  `source` is one of the embedded device sources
@@ -100,6 +102,11 @@ struct twiddles {
 int32_t get_twiddles(int n, const std::vector<int>& radices, bool dbl, std::shared_ptr<twiddles>* out);
 
 bool factorize(int n, std::vector<int>& radices);
+// LDS bytes of the kernel a configuration names (tile, plus the LDS twiddle tables with twl).
+size_t fft_lds_bytes(const fft_config& c);
+// Sets c.twl by the rule of `mode` (0 never, 1 default rule, 2 wherever the kernel has the
+// form) for a stage of these lines, and only where the LDS budget allows it.
+void apply_twl(fft_config& c, const lines_desc& d, int mode);
 // Chooses tpl / lpb for a length and access mode (heuristic default of the autotuner).
 fft_config default_fft_config(int n, bool dbl, bool in_cols, bool out_cols, bool inplace);
 std::vector<fft_config> fft_candidates(int n, bool dbl, bool in_cols, bool out_cols, bool inplace);

@@ -25,7 +25,9 @@ extern "C" int32_t dtfft_hip_kernel_destroy(void*);
 
 namespace dtfft_hip {
 
-static dtfft_hip_config_t g_config = {0, 1, 0, 2, 5, nullptr, 1};
+static dtfft_hip_config_t g_config = {0, 1, 0, 2, 5, nullptr, 1, 1};
+
+const dtfft_hip_config_t& global_config() { return g_config; }
 
 struct permute {
     void* h = nullptr;
@@ -65,10 +67,21 @@ struct plan_t {
     int kind[3] = {0, 0, 0};                         // R2R kinds along X, Y, Z
     r2r_stage rx, ry, rz, rxb, ryb, rzb;             // R2R forward / backward stages
     void* work = nullptr;                            // complex work lines of the R2R stages
+    // forward + backward captured as one hipGraph for one set of pointers
+    hipStream_t gstream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    void* gptr[4] = {nullptr, nullptr, nullptr, nullptr};
 
     plan_t() = default;
     plan_t(const plan_t&) = delete;
     ~plan_t() {
+        if (gexec) hipGraphExecDestroy(gexec);
+        if (graph) hipGraphDestroy(graph);
+        if (gstream) hipStreamDestroy(gstream);
+        if (ev_a) hipEventDestroy(ev_a);
+        if (ev_b) hipEventDestroy(ev_b);
         if (aux) hipFree(aux);
         if (work) hipFree(work);
     }
@@ -397,7 +410,7 @@ extern "C" {
 
 int32_t dtfft_hip_create_config(dtfft_hip_config_t* c) {
     if (!c) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
-    *c = dtfft_hip_config_t{0, 1, 0, 2, 5, nullptr, 1};
+    *c = dtfft_hip_config_t{0, 1, 0, 2, 5, nullptr, 1, 1};
     return DTFFT_HIP_SUCCESS;
 }
 
@@ -406,6 +419,7 @@ int32_t dtfft_hip_set_config(const dtfft_hip_config_t* c) {
     if (c->measure_warmup_iters < 0) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
     if (c->measure_iters <= 0) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
     if (c->lds_dma < 0 || c->lds_dma > 3) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    if (c->twl < 0 || c->twl > 2) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
     g_config = *c;
     return DTFFT_HIP_SUCCESS;
 }
@@ -437,6 +451,49 @@ int32_t dtfft_hip_execute(dtfft_hip_plan_t plan, void* in, void* out, int32_t ex
     if (p->transpose_only && in == out) return DTFFT_HIP_ERROR_INPLACE_TRANSPOSE;
     steps r;
     return exec_steps(p, in, out, fwd, aux, r);
+}
+
+int32_t dtfft_hip_execute_cycle_graph(dtfft_hip_plan_t plan, void* in, void* out, void* aux, void* in2) {
+    auto* p = (plan_t*)plan;
+    if (!p) return DTFFT_HIP_ERROR_PLAN_NOT_CREATED;
+    if (!in || !out || !in2) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    if (out == in || out == in2) return DTFFT_HIP_ERROR_INVALID_USAGE;
+    if (aux && (aux == in || aux == out || aux == in2)) return DTFFT_HIP_ERROR_INVALID_AUX;
+    if (compile_only()) return DTFFT_HIP_ERROR_NO_DEVICE;
+    // the library's aux is allocated before the capture, never inside it
+    TRY(ensure_aux(p, aux));
+    void* ptrs[4] = {in, out, aux, in2};
+    if (!p->gexec || !std::equal(ptrs, ptrs + 4, p->gptr)) {
+        if (!p->gstream) {
+            HIP_TRY(hipStreamCreateWithFlags(&p->gstream, hipStreamNonBlocking));
+            HIP_TRY(hipEventCreateWithFlags(&p->ev_a, hipEventDisableTiming));
+            HIP_TRY(hipEventCreateWithFlags(&p->ev_b, hipEventDisableTiming));
+        }
+        if (p->gexec) { hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+        if (p->graph) { hipGraphDestroy(p->graph); p->graph = nullptr; }
+        // the execute sequences launch on the plan's stream: point it at the capture stream
+        HIP_TRY(hipStreamBeginCapture(p->gstream, hipStreamCaptureModeThreadLocal));
+        const hipStream_t user = p->stream;
+        p->stream = p->gstream;
+        steps r;
+        int32_t rc = exec_steps(p, in, out, true, aux, r);
+        if (rc == DTFFT_HIP_SUCCESS) rc = exec_steps(p, out, in2, false, aux, r);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(p->gstream, &g);
+        p->stream = user;
+        if (rc != DTFFT_HIP_SUCCESS) { if (g) hipGraphDestroy(g); return rc; }
+        if (e != hipSuccess) { set_last_error("hipStreamEndCapture failed"); return DTFFT_HIP_ERROR_HIP_RUNTIME; }
+        p->graph = g;
+        HIP_TRY(hipGraphInstantiate(&p->gexec, p->graph, nullptr, nullptr, 0));
+        std::copy(ptrs, ptrs + 4, p->gptr);
+    }
+    // the replay runs after the work queued on the plan's stream and before what follows it
+    HIP_TRY(hipEventRecord(p->ev_a, p->stream));
+    HIP_TRY(hipStreamWaitEvent(p->gstream, p->ev_a, 0));
+    HIP_TRY(hipGraphLaunch(p->gexec, p->gstream));
+    HIP_TRY(hipEventRecord(p->ev_b, p->gstream));
+    HIP_TRY(hipStreamWaitEvent(p->stream, p->ev_b, 0));
+    return DTFFT_HIP_SUCCESS;
 }
 
 int32_t dtfft_hip_plan_execute_stage(dtfft_hip_plan_t plan, void* in, void* out, int32_t execute_type, void* aux,

@@ -199,6 +199,11 @@ typedef struct {
                                      them (default), 2 the double-buffer kernel wherever the stage shape
                                      allows, 3 the single-buffer kernel (one tile per block, two
                                      blocks per CU) likewise */
+    int32_t twl;                  /* FFT kernels that keep the twiddle table in LDS (persistent blocks):
+                                     0 never, 1 where the default rule picks them (default: FP32 rows
+                                     of n >= 2048, FP32 column-in passes without LDS-DMA, the fused
+                                     C2R, on stages of >= 2048 tiles), 2 every stage whose kernel has
+                                     the form and whose LDS budget allows it */
 } dtfft_hip_config_t;
 
 int32_t dtfft_hip_create_config(dtfft_hip_config_t* config);
@@ -217,6 +222,13 @@ int32_t dtfft_hip_create_plan_r2r(int8_t ndims, const int32_t* dims, const int32
 int32_t dtfft_hip_execute(dtfft_hip_plan_t plan, void* in, void* out, int32_t execute_type, void* aux);
 int32_t dtfft_hip_transpose(dtfft_hip_plan_t plan, void* in, void* out, int32_t transpose_type, void* aux);
 int32_t dtfft_hip_destroy(dtfft_hip_plan_t* plan);
+/* Forward then backward of one plan, captured once as a hipGraph and replayed (no counterpart
+   in the reference, which launches every kernel of execute FORWARD / BACKWARD itself):
+   in -> out (forward), out -> in2 (backward, unnormalised), aux as in dtfft_hip_execute (NULL:
+   the plan's own, allocated before the capture).  The graph is captured again when the
+   pointers change.  Ordered on the plan's stream like two execute calls; the backward may
+   overwrite out, as execute BACKWARD may. */
+int32_t dtfft_hip_execute_cycle_graph(dtfft_hip_plan_t plan, void* in, void* out, void* aux, void* in2);
 /* starts/counts in reverse order, as dtfft_get_local_sizes; alloc_size in elements. */
 int32_t dtfft_hip_get_local_sizes(dtfft_hip_plan_t plan, int32_t* in_starts, int32_t* in_counts,
                                   int32_t* out_starts, int32_t* out_counts, size_t* alloc_size);

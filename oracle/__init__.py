@@ -82,6 +82,8 @@ def _declare(L):
         "orc_permute_forward_read": [vp, vp, i, i, ip],
         "orc_permute_backward_write": [vp, vp, i, ip],
         "orc_permute_backward_read": [vp, vp, i, ip],
+        "orc_permute_forward_block": [vp, vp, i, i, ip, i, i],
+        "orc_permute_backward_block": [vp, vp, i, ip, i, i],
         "orc_permute_backward_start_write": [vp, vp, i, ip],
         "orc_permute_backward_start_read": [vp, vp, i, ip],
         "orc_permute_backward_end_pipelined_write": [vp, vp, i, ip, ip],
@@ -226,6 +228,64 @@ def host_kernel(kernel_type: int, dims, inbuf: np.ndarray, outbuf: np.ndarray,
             f(ip, op, es, d, loc(neighbor))
         else:
             raise ValueError(f"unknown kernel type {kernel_type}")
+
+
+# ---------------------------------------------------------------------------
+# Host kernel variants and their timed selection (kernel_host%create_host)
+# ---------------------------------------------------------------------------
+BLOCK_SIZES = (4, 8, 16, 32, 64)   # HOST_KERNEL_BLOCK_4 .. HOST_KERNEL_BLOCK_64
+
+
+def permute_variant(kernel_type: int, dims, inbuf: np.ndarray, outbuf: np.ndarray, block: int = 0,
+                    access: str = "write") -> None:
+    """PERMUTE_FORWARD / PERMUTE_BACKWARD with a host-kernel variant: block 0 is BASE
+    (_dtfft_kernel_host_routines.inc), block 4..64 the cache-blocked BLOCK_<block> routines
+    (_dtfft_kernel_host_block_routines.inc); access "write" or "read" loop order."""
+    dims = [int(d) for d in dims]
+    kt = effective_kernel_type(kernel_type, len(dims))
+    if block == 0:
+        host_kernel(kt, dims, inbuf, outbuf, access=access)
+        return
+    es = inbuf.itemsize
+    d = _iarr(dims + [1] * (3 - len(dims)))
+    read = 1 if access == "read" else 0
+    if kt == KERNEL_PERMUTE_FORWARD:
+        lib().orc_permute_forward_block(_ptr(inbuf), _ptr(outbuf), es, len(dims), d, int(block), read)
+    elif kt == KERNEL_PERMUTE_BACKWARD:
+        lib().orc_permute_backward_block(_ptr(inbuf), _ptr(outbuf), es, d, int(block), read)
+    else:
+        raise ValueError(f"no blocked variant restated for kernel type {kernel_type}")
+
+
+def autotune_permute(kernel_type: int, dims, es: int, warmup: int = 2, iters: int = 5):
+    """Timed choice of a permute's host kernel, as kernel_host%create_host does at
+    DTFFT_EXHAUSTIVE (src/dtfft_kernel_host.F90:178-291): BASE is timed with WRITE and with
+    READ access (select_access_mode, _dtfft_kernel_host_routines.inc:98-135) and keeps the
+    faster mode; BLOCK_4 .. BLOCK_64 are then timed in that mode; the fastest wins.  Warm-up
+    and timed iterations default to the reference's measure_warmup_iters / measure_iters.
+    Returns (block, access) with block 0 for BASE."""
+    import time
+    n = int(np.prod([int(d) for d in dims]))
+    dt = elem_dtype(es)
+    a = np.zeros(n, dtype=dt)
+    b = np.zeros(n, dtype=dt)
+
+    def bench(block, access):
+        for _ in range(warmup):
+            permute_variant(kernel_type, dims, a, b, block, access)
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            permute_variant(kernel_type, dims, a, b, block, access)
+        return (time.perf_counter() - t0) / iters
+
+    tw, tr = bench(0, "write"), bench(0, "read")
+    access = "write" if tw < tr else "read"
+    best, best_t = 0, min(tw, tr)
+    for blk in BLOCK_SIZES:
+        t = bench(blk, access)
+        if t < best_t:
+            best, best_t = blk, t
+    return best, access
 
 
 def elem_dtype(es: int):

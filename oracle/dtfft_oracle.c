@@ -127,6 +127,90 @@ void orc_permute_backward_read(const void* vin, void* vout, int es, const int* d
     });
 }
 
+/* Cache-blocked permutes, _dtfft_kernel_host_block_routines.inc (one routine per BLOCK_SIZE
+ * in the reference, 4..64, instantiated from the same text; `block` is that size here).
+ * Same index maps as the BASE routines above; the loops visit block x block (x block) tiles.
+ * permute_forward_write :91-144, permute_forward_read :149-202. */
+void orc_permute_forward_block(const void* vin, void* vout, int es, int ndims, const int* dims, int block,
+                               int read)
+{
+    const long B = block, nx = dims[0], ny = dims[1];
+#define ORC_MIN(a, b) ((a) < (b) ? (a) : (b))
+    if (ndims == 2) {
+        ES_DISPATCH(es, {
+            const T* in = (const T*)vin; T* out = (T*)vout;
+            if (!read) {
+                _Pragma("omp parallel for collapse(2)")
+                for (long xb = 0; xb < nx; xb += B)
+                    for (long yb = 0; yb < ny; yb += B)
+                        for (long x = xb; x < ORC_MIN(xb + B, nx); x++)
+                            for (long y = yb; y < ORC_MIN(yb + B, ny); y++)
+                                out[x * ny + y] = in[y * nx + x];
+            } else {
+                _Pragma("omp parallel for collapse(2)")
+                for (long yb = 0; yb < ny; yb += B)
+                    for (long xb = 0; xb < nx; xb += B)
+                        for (long y = yb; y < ORC_MIN(yb + B, ny); y++)
+                            for (long x = xb; x < ORC_MIN(xb + B, nx); x++)
+                                out[x * ny + y] = in[y * nx + x];
+            }
+        });
+        return;
+    }
+    const long nz = dims[2], nx_ny = nx * ny, ny_nz = ny * nz;
+    ES_DISPATCH(es, {
+        const T* in = (const T*)vin; T* out = (T*)vout;
+        if (!read) {
+            _Pragma("omp parallel for collapse(3)")
+            for (long xb = 0; xb < nx; xb += B)
+                for (long zb = 0; zb < nz; zb += B)
+                    for (long yb = 0; yb < ny; yb += B)
+                        for (long x = xb; x < ORC_MIN(xb + B, nx); x++)
+                            for (long z = zb; z < ORC_MIN(zb + B, nz); z++)
+                                for (long y = yb; y < ORC_MIN(yb + B, ny); y++)
+                                    out[x * ny_nz + z * ny + y] = in[z * nx_ny + y * nx + x];
+        } else {
+            _Pragma("omp parallel for collapse(3)")
+            for (long zb = 0; zb < nz; zb += B)
+                for (long yb = 0; yb < ny; yb += B)
+                    for (long xb = 0; xb < nx; xb += B)
+                        for (long z = zb; z < ORC_MIN(zb + B, nz); z++)
+                            for (long y = yb; y < ORC_MIN(yb + B, ny); y++)
+                                for (long x = xb; x < ORC_MIN(xb + B, nx); x++)
+                                    out[x * ny_nz + z * ny + y] = in[z * nx_ny + y * nx + x];
+        }
+    });
+}
+
+/* permute_backward_write :207-239, permute_backward_read :244-276 (blocked). */
+void orc_permute_backward_block(const void* vin, void* vout, int es, const int* dims, int block, int read)
+{
+    const long B = block, nx = dims[0], ny = dims[1], nz = dims[2], nx_ny = nx * ny, nz_nx = nz * nx;
+    ES_DISPATCH(es, {
+        const T* in = (const T*)vin; T* out = (T*)vout;
+        if (!read) {
+            _Pragma("omp parallel for collapse(3)")
+            for (long yb = 0; yb < ny; yb += B)
+                for (long xb = 0; xb < nx; xb += B)
+                    for (long zb = 0; zb < nz; zb += B)
+                        for (long y = yb; y < ORC_MIN(yb + B, ny); y++)
+                            for (long x = xb; x < ORC_MIN(xb + B, nx); x++)
+                                for (long z = zb; z < ORC_MIN(zb + B, nz); z++)
+                                    out[y * nz_nx + x * nz + z] = in[z * nx_ny + y * nx + x];
+        } else {
+            _Pragma("omp parallel for collapse(3)")
+            for (long zb = 0; zb < nz; zb += B)
+                for (long yb = 0; yb < ny; yb += B)
+                    for (long xb = 0; xb < nx; xb += B)
+                        for (long z = zb; z < ORC_MIN(zb + B, nz); z++)
+                            for (long y = yb; y < ORC_MIN(yb + B, ny); y++)
+                                for (long x = xb; x < ORC_MIN(xb + B, nx); x++)
+                                    out[y * nz_nx + x * nz + z] = in[z * nx_ny + y * nx + x];
+        }
+    });
+#undef ORC_MIN
+}
+
 /* permute_backward_start_write, _dtfft_kernel_host_routines.inc:351-390.
  * out(z + y*nz + x*nz*ny) = in(x + y*nx + z*nx*ny). */
 void orc_permute_backward_start_write(const void* vin, void* vout, int es, const int* dims)

@@ -17,17 +17,24 @@ from __future__ import annotations
 import numpy as np
 
 from . import (FFT_BACKWARD, FFT_FORWARD, KERNEL_PERMUTE_BACKWARD, KERNEL_PERMUTE_FORWARD,
-               fft_c2c, fft_c2r, fft_r2c, host_kernel)
+               autotune_permute, fft_c2c, fft_c2r, fft_r2c, permute_variant)
 
 
-def _permute(kernel_type, a: np.ndarray, fdims) -> np.ndarray:
-    """Apply a permute kernel to a pencil whose Fortran dims are ``fdims``."""
+def _kernel_key(kernel_type, fdims) -> str:
+    name = "PERMUTE_FORWARD" if kernel_type == KERNEL_PERMUTE_FORWARD else "PERMUTE_BACKWARD"
+    return name + " " + "x".join(str(int(d)) for d in fdims)
+
+
+def _permute(kernel_type, a: np.ndarray, fdims, kernels=None) -> np.ndarray:
+    """Apply a permute kernel to a pencil whose Fortran dims are ``fdims``; ``kernels`` maps
+    "<kind> <dims>" to the host-kernel variant (block, access) to run (default BASE/WRITE)."""
     fdims = [int(d) for d in fdims]
     flat = np.ascontiguousarray(a).ravel()
     out = np.empty_like(flat)
     view_in = flat.view(_raw(flat.itemsize))
     view_out = out.view(_raw(flat.itemsize))
-    host_kernel(kernel_type, fdims, view_in, view_out)
+    block, access = (kernels or {}).get(_kernel_key(kernel_type, fdims), (0, "write"))
+    permute_variant(kernel_type, fdims, view_in, view_out, block, access)
     if len(fdims) == 2:
         shape = (fdims[0], fdims[1])  # out Fortran [b, a] -> numpy (a, b)
     elif kernel_type == KERNEL_PERMUTE_FORWARD:
@@ -53,22 +60,38 @@ def _fft_2d(a, sign):
     return np.swapaxes(t, -1, -2).copy()
 
 
-def execute_generic(x: np.ndarray, forward: bool = True) -> np.ndarray:
+def execute_generic(x: np.ndarray, forward: bool = True, kernels=None) -> np.ndarray:
     """3D C2C generic cycle on one rank. Forward input is the X-pencil (nz, ny, nx);
-    output the Z-pencil (ny, nx, nz).  Backward is the reverse."""
+    output the Z-pencil (ny, nx, nz).  Backward is the reverse.  ``kernels``: host-kernel
+    variants of the permutes (from autotune_generic); default BASE/WRITE."""
+    k = kernels
     if forward:
         nz, ny, nx = x.shape
         a = _fft_last(x, FFT_FORWARD)
-        a = _permute(KERNEL_PERMUTE_FORWARD, a, (nx, ny, nz))       # X->Y: (nx, nz, ny)
+        a = _permute(KERNEL_PERMUTE_FORWARD, a, (nx, ny, nz), k)    # X->Y: (nx, nz, ny)
         a = _fft_last(a, FFT_FORWARD)
-        a = _permute(KERNEL_PERMUTE_FORWARD, a, (ny, nz, nx))       # Y->Z: (ny, nx, nz)
+        a = _permute(KERNEL_PERMUTE_FORWARD, a, (ny, nz, nx), k)    # Y->Z: (ny, nx, nz)
         return _fft_last(a, FFT_FORWARD)
     ny, nx, nz = x.shape
     a = _fft_last(x, FFT_BACKWARD)
-    a = _permute(KERNEL_PERMUTE_BACKWARD, a, (nz, nx, ny))          # Z->Y: (nx, nz, ny)
+    a = _permute(KERNEL_PERMUTE_BACKWARD, a, (nz, nx, ny), k)       # Z->Y: (nx, nz, ny)
     a = _fft_last(a, FFT_BACKWARD)
-    a = _permute(KERNEL_PERMUTE_BACKWARD, a, (ny, nz, nx))          # Y->X: (nz, ny, nx)
+    a = _permute(KERNEL_PERMUTE_BACKWARD, a, (ny, nz, nx), k)       # Y->X: (nz, ny, nx)
     return _fft_last(a, FFT_BACKWARD)
+
+
+def autotune_generic(dims, es: int, warmup: int = 2, iters: int = 5) -> dict:
+    """Host-kernel variant of each permute of the 3D generic cycle of Fortran dims
+    (nx, ny, nz), picked by timing as kernel_host%create_host does at DTFFT_EXHAUSTIVE
+    (src/dtfft_kernel_host.F90:178-291).  Returns {"<kind> <dims>": (block, access)}."""
+    nx, ny, nz = [int(d) for d in dims]
+    out = {}
+    for kt, fd in ((KERNEL_PERMUTE_FORWARD, (nx, ny, nz)), (KERNEL_PERMUTE_FORWARD, (ny, nz, nx)),
+                   (KERNEL_PERMUTE_BACKWARD, (nz, nx, ny)), (KERNEL_PERMUTE_BACKWARD, (ny, nz, nx))):
+        key = _kernel_key(kt, fd)
+        if key not in out:
+            out[key] = autotune_permute(kt, fd, es, warmup, iters)
+    return out
 
 
 def execute_z_slab(x: np.ndarray, forward: bool = True) -> np.ndarray:

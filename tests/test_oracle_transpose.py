@@ -187,3 +187,43 @@ def test_empty_dims_is_dummy():
     inp = payload(10, 4)
     out = run(O.KERNEL_PERMUTE_FORWARD, (10, 0, 1), inp)
     assert (out == 0).all()
+
+
+@pytest.mark.parametrize("es", ES)
+@pytest.mark.parametrize("dims", SHAPES3 + SHAPES2)
+def test_blocked_host_variants_against_numpy(es, dims):
+    """The cache-blocked BLOCK_4..64 host permutes (_dtfft_kernel_host_block_routines.inc),
+    in both loop orders, give the numpy axis permutation for every block size, including
+    blocks that do not divide the extents."""
+    inp = payload(int(np.prod(dims)), es)
+    if len(dims) == 3:
+        a = f3(inp, dims)
+        want = {O.KERNEL_PERMUTE_FORWARD: np.transpose(a, (2, 0, 1)).ravel(),
+                O.KERNEL_PERMUTE_BACKWARD: np.transpose(a, (1, 2, 0)).ravel()}
+    else:
+        a = inp.reshape(dims[1], dims[0])
+        want = {O.KERNEL_PERMUTE_FORWARD: a.T.ravel()}
+    for kt, w in want.items():
+        for block in O.BLOCK_SIZES:
+            for access in ("write", "read"):
+                out = np.zeros_like(inp)
+                O.permute_variant(kt, dims, inp, out, block, access)
+                assert np.array_equal(out, w), (kt, block, access)
+
+
+def test_host_autotune_picks_a_listed_variant_that_is_exact():
+    """The timed selection (kernel_host%create_host at EXHAUSTIVE) returns BASE or a BLOCK
+    size with an access mode, and the plan-level cycle run with the picked variants equals
+    the BASE run bit for bit."""
+    import oracle.plan as OP
+    dims = (33, 20, 14)
+    tuned = OP.autotune_generic(dims, 8, warmup=1, iters=1)
+    assert len(tuned) == 4
+    for block, access in tuned.values():
+        assert block in (0,) + O.BLOCK_SIZES and access in ("write", "read")
+    rng = np.random.default_rng(0)
+    x = (rng.uniform(-1, 1, dims[::-1]) + 1j * rng.uniform(-1, 1, dims[::-1])).astype(np.complex64)
+    z0 = OP.execute_generic(x, True)
+    z1 = OP.execute_generic(x, True, kernels=tuned)
+    assert z0.tobytes() == z1.tobytes()
+    assert OP.execute_generic(z0, False, kernels=tuned).tobytes() == OP.execute_generic(z0, False).tobytes()
