@@ -181,12 +181,15 @@ template <typename T> struct bfly<T, 13> : bfly_prime<T, 13> {};
 // ---------------------------------------------------------------------------
 // Radix schedule (compile time).
 // ---------------------------------------------------------------------------
-template <int N_, int R0, int R1 = 1, int R2 = 1, int R3 = 1, int R4 = 1, int R5 = 1, int R6 = 1>
+template <int N_, int R0, int R1 = 1, int R2 = 1, int R3 = 1, int R4 = 1, int R5 = 1, int R6 = 1, int R7 = 1,
+          int R8 = 1, int R9 = 1>
 struct sched {
     static constexpr int N = N_;
-    static constexpr int NP = (R0 > 1) + (R1 > 1) + (R2 > 1) + (R3 > 1) + (R4 > 1) + (R5 > 1) + (R6 > 1);
+    static constexpr int NP = (R0 > 1) + (R1 > 1) + (R2 > 1) + (R3 > 1) + (R4 > 1) + (R5 > 1) + (R6 > 1) + (R7 > 1) +
+                              (R8 > 1) + (R9 > 1);
     static constexpr __device__ __host__ int radix(int p) {
-        return p == 0 ? R0 : p == 1 ? R1 : p == 2 ? R2 : p == 3 ? R3 : p == 4 ? R4 : p == 5 ? R5 : R6;
+        return p == 0 ? R0 : p == 1 ? R1 : p == 2 ? R2 : p == 3 ? R3 : p == 4 ? R4 : p == 5 ? R5 : p == 6 ? R6
+               : p == 7 ? R7 : p == 8 ? R8 : R9;
     }
     static constexpr __device__ __host__ int span(int p) {  // L_p = product of radices before p
         return p == 0 ? 1 : span(p - 1) * radix(p - 1);

@@ -36,11 +36,18 @@ bool factorize(int n, std::vector<int>& radices) {
     for (int p : primes)
         while (m % p == 0) { radices.push_back(p); m /= p; }
     if (m != 1) return false;
-    if (radices.size() > 7) return false;
+    if (radices.size() > 10) return false;  // dtfft::sched holds up to 10 passes
     return true;
 }
 
 std::string fft_config::expr() const {
+    if (any) {
+        std::string s = std::string("fft_any_kernel<") + (dbl ? "double" : "float") + ", dtfft::sched<" +
+                        std::to_string(n);
+        for (int r : radices) s += ", " + std::to_string(r);
+        return s + ">, " + std::to_string(tpl) + ", " + std::to_string(lpb) + ", " + std::to_string(pad) + ", " +
+               std::to_string(kind) + (chirp ? ", true>" : ", false>");
+    }
     if (dma) {
         std::string s = std::string(dma == 2 ? "fft_cols_dma1_kernel<" : "fft_cols_dma_kernel<") +
                         (dbl ? "double" : "float") + ", dtfft::sched<" + std::to_string(n);
@@ -101,6 +108,7 @@ static size_t tw_entries(const fft_config& c) {
 
 size_t fft_lds_bytes(const fft_config& c) {
     if (c.dma) return lds_dma(c);
+    if (c.any) return (size_t)c.lpb * lds_line(c.n, c.pad) * elem_bytes(c.dbl);
     size_t b = (size_t)c.lpb * lds_line(c.n, c.pad) * elem_bytes(c.dbl);
     if (c.twl) {
         // the tables are separate __shared__ arrays of 16-byte entries (FP32 hi+lo pairs or
@@ -121,7 +129,7 @@ static bool valid(const fft_config& c) {
 void apply_twl(fft_config& c, const lines_desc& d, int mode) {
     c.twl = 0;
     // the LDS-DMA column kernels keep their twiddles in LDS anyway; R2C rows have no such form
-    if (mode == 0 || c.dma || c.kind == 1 || c.n < 2) return;
+    if (mode == 0 || c.dma || c.any || c.kind == 1 || c.n < 2) return;
     const int64_t ntiles = ((d.nl + c.lpb - 1) / c.lpb) * d.nb;
     bool want = mode >= 2;
     if (mode == 1) {
@@ -200,6 +208,44 @@ std::vector<fft_config> fft_candidates(int n, bool dbl, bool in_cols, bool out_c
     return out;
 }
 
+// Any-length kernel: the work line of m points sits whole in LDS.  Bluestein takes the
+// smallest m >= 2n - 1 whose primes are <= 7.  One butterfly of the
+// largest radix per thread and pass; lines per block grow to about 256 threads while the
+// tile stays within 64 KiB of LDS, so that several blocks share a CU.
+bool any_config(int kind, int n, bool dbl, bool direct, fft_config& c) {
+    c = fft_config{};
+    c.any = 1;
+    c.kind = kind;
+    c.nlog = n;
+    c.dbl = dbl;
+    c.chirp = !direct;
+    const int64_t es = elem_bytes(dbl);
+    const int64_t mmax = (int64_t)(kMaxLds / (size_t)es);
+    int64_t m = direct ? n : 2 * (int64_t)n - 1;
+    for (;; m++) {
+        if (m > mmax) return false;
+        if (!factorize((int)m, c.radices)) {
+            if (direct) return false;
+            continue;
+        }
+        if (direct) break;
+        bool small = true;
+        for (int r : c.radices) small &= (r == 1 || r == 2 || r == 3 || r == 4 || r == 5 || r == 7 || r == 8 || r == 16);
+        if (small) break;
+    }
+    c.n = (int)m;
+    const int rmax = *std::max_element(c.radices.begin(), c.radices.end());
+    c.tpl = std::max<int>(1, (int)(m / rmax));
+    while (c.tpl > 1024) c.tpl = (c.tpl + 1) / 2;
+    c.lpb = 1;
+    c.pad = 0;
+    while (c.lpb * c.tpl < 256 && c.lpb * 2 * c.tpl <= 1024 &&
+           (size_t)(c.lpb * 2) * lds_line(c.n, 1) * (size_t)es <= 64 * 1024)
+        c.lpb *= 2;
+    if (c.lpb > 1) c.pad = 1;
+    return valid(c);
+}
+
 // ---- twiddles -------------------------------------------------------------------------
 twiddles::~twiddles() {
     // Tables live in a process-wide cache and are released at exit; freeing device memory
@@ -266,6 +312,36 @@ int32_t get_rtwiddles(int n, bool dbl, std::shared_ptr<twiddles>* out) {
     return DTFFT_HIP_SUCCESS;
 }
 
+int32_t get_bluestein(int n, int m, bool dbl, std::shared_ptr<twiddles>* chirp, std::shared_ptr<twiddles>* bh) {
+    const std::string kc = "c" + std::to_string(n) + (dbl ? "d" : "f");
+    const std::string kb = "b" + std::to_string(n) + "," + std::to_string(m) + (dbl ? "d" : "f");
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    auto ic = g_tw.find(kc), ib = g_tw.find(kb);
+    if (ic != g_tw.end() && ib != g_tw.end()) { *chirp = ic->second; *bh = ib->second; return DTFFT_HIP_SUCCESS; }
+    const std::string t = dbl ? "double" : "float";
+    const rtc_kernel *kch = nullptr, *kbh = nullptr;
+    TRY(rtc_get("anylen", "gen_chirp<" + t + ">", &kch));
+    TRY(rtc_get("anylen", "gen_bluestein_bh<" + t + ">", &kbh));
+    auto c = std::make_shared<twiddles>(), b = std::make_shared<twiddles>();
+    c->bytes = (int64_t)n * 16;
+    b->bytes = (int64_t)m * 16;
+    if (!compile_only()) {
+        HIP_TRY(hipMalloc(&c->ptr, (size_t)c->bytes));
+        HIP_TRY(hipMalloc(&b->ptr, (size_t)b->bytes));
+        int nn = n, mm = m;
+        void* ac[] = {&c->ptr, &nn};
+        TRY(launch(kch, (unsigned)((n + 255) / 256), 256, nullptr, ac));
+        void* ab[] = {&b->ptr, &nn, &mm};
+        TRY(launch(kbh, (unsigned)m, 256, nullptr, ab));
+        HIP_TRY(hipDeviceSynchronize());
+    }
+    g_tw[kc] = c;
+    g_tw[kb] = b;
+    *chirp = c;
+    *bh = b;
+    return DTFFT_HIP_SUCCESS;
+}
+
 // ---- stages ---------------------------------------------------------------------------
 lines_desc make_lines(int64_t nl, int64_t nb, int64_t in_ls, int64_t in_bs, int64_t in_ns, int64_t out_ls,
                       int64_t out_bs, int64_t out_ns, int lpb) {
@@ -283,6 +359,14 @@ int32_t fft_stage::init(const fft_config& c, const lines_desc& desc) {
     d = desc;
     d.ntl = (d.nl + c.lpb - 1) / c.lpb;
     d.ntiles = d.ntl * d.nb;
+    if (c.any) {
+        TRY(rtc_get("anylen", c.expr(), &k));
+        TRY(get_twiddles(c.n, c.radices, c.dbl, &tw));
+        if (c.chirp) TRY(get_bluestein(c.nlog, c.n, c.dbl, &chirp, &bh));
+        block = (unsigned)(c.tpl * c.lpb);
+        grid = (unsigned)std::min<int64_t>(d.ntiles, 1 << 22);
+        return DTFFT_HIP_SUCCESS;
+    }
     if (c.n > 1 || c.kind != 0) {
         TRY(rtc_get("fft", c.expr(), &k));
         TRY(get_twiddles(c.n, c.radices, c.dbl, &tw));
@@ -330,6 +414,15 @@ int32_t fft_stage::init(const fft_config& c, const lines_desc& desc) {
 
 int32_t fft_stage::run(const void* in, void* out, int sign, hipStream_t s) const {
     if (d.ntiles == 0) return DTFFT_HIP_SUCCESS;
+    if (cfg.any) {
+        void* twp = tw ? tw->ptr : nullptr;
+        void* cp = chirp ? chirp->ptr : nullptr;
+        void* bp = bh ? bh->ptr : nullptr;
+        lines_desc dd = d;
+        int inv = sign > 0 ? 1 : 0, nl = cfg.nlog;
+        void* args[] = {(void*)&in, &out, &twp, &cp, &bp, &dd, &inv, &nl};
+        return launch(k, grid, block, s, args);
+    }
     if (cfg.kind != 0) {
         void* twp = tw ? tw->ptr : nullptr;
         void* rtwp = rtw ? rtw->ptr : nullptr;
@@ -382,8 +475,13 @@ int32_t plan_fft_stage(int n, bool dbl, int64_t nl, int64_t nb, int64_t in_ls, i
                        hipStream_t stream, fft_stage* st, int lds_dma) {
     std::vector<int> rad;
     if (!factorize(n, rad)) {
-        set_last_error("FFT length " + std::to_string(n) + " has a prime factor > 13");
-        return DTFFT_HIP_ERROR_UNSUPPORTED_SIZE;
+        // a prime factor above 13, or more passes than a schedule holds: Bluestein
+        fft_config c;
+        if (!any_config(0, n, dbl, false, c)) {
+            set_last_error("FFT length " + std::to_string(n) + ": the Bluestein work line does not fit in LDS");
+            return DTFFT_HIP_ERROR_UNSUPPORTED_SIZE;
+        }
+        return st->init(c, make_lines(nl, nb, in_ls, in_bs, in_ns, out_ls, out_bs, out_ns, c.lpb));
     }
     const bool in_cols = (in_ns != 1 && in_ls == 1);
     const bool out_cols = (out_ns != 1 && out_ls == 1);
@@ -478,15 +576,22 @@ int32_t plan_fft_stage(int n, bool dbl, int64_t nl, int64_t nb, int64_t in_ls, i
 int32_t plan_real_stage(int kind, int n, bool dbl, int64_t how_many, int64_t idist, int64_t odist, int effort,
                         hipStream_t stream, fft_stage* st) {
     (void)effort; (void)stream;
-    if (n < 2 || n % 2 != 0) {
-        set_last_error("R2C/C2R stages need an even length, got " + std::to_string(n));
-        return DTFFT_HIP_ERROR_UNSUPPORTED_SIZE;
+    if (n < 1) {
+        set_last_error("R2C/C2R length must be positive, got " + std::to_string(n));
+        return DTFFT_HIP_ERROR_INVALID_DIMENSION_SIZE;
     }
     const int m = n / 2;
     fft_config c;
-    if (!factorize(m, c.radices)) {
-        set_last_error("R2C length " + std::to_string(n) + ": n/2 has a prime factor > 13");
-        return DTFFT_HIP_ERROR_UNSUPPORTED_SIZE;
+    if (n % 2 != 0 || !factorize(m, c.radices)) {
+        // odd lengths (directly when n factorizes) and halves with a prime factor above 13
+        // (Bluestein) take the any-length kernel: real side in real units
+        std::vector<int> rad;
+        const bool direct = n % 2 != 0 && factorize(n, rad);
+        if (!any_config(kind, n, dbl, direct, c)) {
+            set_last_error("R2C/C2R length " + std::to_string(n) + ": the work line does not fit in LDS");
+            return DTFFT_HIP_ERROR_UNSUPPORTED_SIZE;
+        }
+        return st->init(c, make_lines(how_many, 1, idist, 0, 1, odist, 0, 1, c.lpb));
     }
     c = default_fft_config(m, dbl, false, false, false);
     c.kind = kind;
@@ -519,8 +624,14 @@ int32_t plan_c2r_cols_stage(int n, bool dbl, int64_t nl, int64_t nb, int64_t in_
     const int m = n / 2;
     fft_config c;
     if (!factorize(m, c.radices) || m < 2) {
-        set_last_error("fused C2R length " + std::to_string(n) + ": n/2 must be > 1 with prime factors <= 13");
-        return DTFFT_HIP_ERROR_UNSUPPORTED_SIZE;
+        // Bluestein (or the direct form for n = 2) on the any-length kernel, which writes
+        // real units: the strides of the complex view double
+        std::vector<int> rad;
+        if (!any_config(2, n, dbl, factorize(n, rad) && m < 2, c)) {
+            set_last_error("fused C2R length " + std::to_string(n) + ": the work line does not fit in LDS");
+            return DTFFT_HIP_ERROR_UNSUPPORTED_SIZE;
+        }
+        return st->init(c, make_lines(nl, nb, 1, in_bs, in_ns, 2 * out_ls, 2 * out_bs, 1, c.lpb));
     }
     c = default_fft_config(m, dbl, true, false, false);
     c.kind = 2;

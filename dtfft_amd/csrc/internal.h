@@ -90,6 +90,12 @@ struct fft_config {
     int dma = 0;  // column-in passes through LDS filled by direct loads: 1 double buffer
                   // (fft_cols_dma_kernel), 2 single buffer, two blocks per CU (fft_cols_dma1_kernel)
     int twl = 0;  // C2C (kind 0, dma 0): persistent blocks with the twiddle table in LDS
+    // fft_any_kernel (device/anylen.hip): any == 1 runs a line of logical length nlog through
+    // the n-point schedule in LDS, directly (chirp false, n == nlog) or by Bluestein's chirp
+    // z-transform (chirp true, n >= 2 nlog - 1); kind 0 / 1 / 2 = C2C / R2C / C2R of nlog
+    int any = 0;
+    int nlog = 0;
+    bool chirp = false;
     std::string expr() const;
 };
 
@@ -100,6 +106,11 @@ struct twiddles {
     ~twiddles();
 };
 int32_t get_twiddles(int n, const std::vector<int>& radices, bool dbl, std::shared_ptr<twiddles>* out);
+// Bluestein tables of logical length n with work length m (device/anylen.hip).
+int32_t get_bluestein(int n, int m, bool dbl, std::shared_ptr<twiddles>* chirp, std::shared_ptr<twiddles>* bh);
+// Configuration of fft_any_kernel for kind 0 / 1 / 2 of logical length n (Bluestein unless
+// `direct`, which needs n to factorize); false when the work line does not fit in LDS.
+bool any_config(int kind, int n, bool dbl, bool direct, fft_config& c);
 
 bool factorize(int n, std::vector<int>& radices);
 // LDS bytes of the kernel a configuration names (tile, plus the LDS twiddle tables with twl).
@@ -119,6 +130,7 @@ struct fft_stage {
     const rtc_kernel* k_alt = nullptr;  // cfg.dma: the plain column kernel, for misaligned buffers
     std::shared_ptr<twiddles> tw;
     std::shared_ptr<twiddles> rtw;  // R2C / C2R untangling twiddles exp(-2 pi i k / 2M)
+    std::shared_ptr<twiddles> chirp, bh;  // cfg.chirp: exp(-i pi j^2 / nlog), DFT_n(b) / n
     unsigned grid = 0, block = 0, grid_alt = 0;
     int32_t init(const fft_config& c, const lines_desc& desc);
     int32_t run(const void* in, void* out, int sign, hipStream_t s) const;

@@ -50,17 +50,15 @@ int32_t r2r_stage::init(int kind_, int64_t n_, bool dbl_, int64_t nl, int64_t nb
     const std::string t = dbl ? "double" : "float";
     // types 3 (DCT-3, DST-3) run the backward transform
     sign = (kind == 4 || kind == 8) ? DTFFT_HIP_FFT_BACKWARD : DTFFT_HIP_FFT_FORWARD;
+    // A work length with a prime factor above 13 runs the three launches, whose C2C stage
+    // is then the Bluestein form of plan_fft_stage.
     std::vector<int> rad;
-    if (!factorize((int)m, rad)) {
-        set_last_error("R2R kind " + std::to_string(kind) + " of length " + std::to_string(n) +
-                       " needs a complex FFT of length " + std::to_string(m) + ", which has a prime factor > 13");
-        return DTFFT_HIP_ERROR_UNSUPPORTED_SIZE;
-    }
+    const bool factors = factorize((int)m, rad);
     // One kernel for unit-stride, evenly spaced lines whose output overwrites nothing but the
     // line itself: a block reads its lines whole before writing them, so in place is safe.
     // Batches that continue the line spacing fold into one range of lines.
     const bool even = nb == 1 || (in_bs == nl * in_ls && out_bs == nl * out_ls);
-    if (in_es == 1 && out_es == 1 && in_ls == out_ls && in_bs == out_bs && even && m > 1) {
+    if (factors && in_es == 1 && out_es == 1 && in_ls == out_ls && in_bs == out_bs && even && m > 1) {
         fft_config c = default_fft_config((int)m, dbl, false, false, false);
         {
             // one butterfly of the largest radix per thread and pass

@@ -125,6 +125,7 @@ static const char* source_text(const std::string& name) {
     if (name == "fft") return k_src_fft;
     if (name == "transpose") return k_src_transpose;
     if (name == "r2r") return k_src_r2r;
+    if (name == "anylen") return k_src_anylen;
     return nullptr;
 }
 
@@ -146,7 +147,7 @@ static int32_t compile(const std::string& src_name, const std::string& expr, std
         set_last_error("unknown device source " + src_name);
         return DTFFT_HIP_ERROR_RTC_COMPILE;
     }
-    // r2r.hip includes fft.hip for the passes of its fused kernel
+    // r2r.hip and anylen.hip include fft.hip for the passes of fft_tile
     const char* hdrs[] = {k_src_common, k_src_fft};
     const char* hdr_names[] = {"common.hip", "fft.hip"};
     hiprtcProgram prog;
@@ -192,7 +193,7 @@ int32_t rtc_get(const std::string& source, const std::string& expr, const rtc_ke
     const char* src = source_text(source);
     if (!src) { set_last_error("unknown source " + source); return DTFFT_HIP_ERROR_RTC_COMPILE; }
     std::string h = source + "|" + expr + "|" + std::string(src) + "|" + std::string(k_src_common);
-    if (source == "r2r") h += "|" + std::string(k_src_fft);
+    if (source == "r2r" || source == "anylen") h += "|" + std::string(k_src_fft);
     for (auto& o : rtc_options()) h += "|" + o;
     char hex[32];
     std::snprintf(hex, sizeof hex, "%016llx", (unsigned long long)fnv1a(h));

@@ -168,8 +168,11 @@ int32_t dtfft_hip_kernel_destroy(void* handle);
  * r2r_kinds is required for FFT_R2R (dtFFT kind ids, include/dtfft_config.h.in:66-73), one per
  * dimension in fft_sizes order; FFTW semantics, unnormalised.  The forward sign applies the
  * kinds, the backward sign their inverses (src/dtfft_utils.F90:447-471).  An R2R stage runs
- * a complex FFT of length n, 2n (type 4), 2(n-1) (DCT-1) or 2(n+1) (DST-1), which must
- * factor into primes <= 13 (DTFFT_HIP_ERROR_UNSUPPORTED_SIZE otherwise). */
+ * a complex FFT of length n, 2n (type 4), 2(n-1) (DCT-1) or 2(n+1) (DST-1).
+ * Any length is accepted, as cuFFT and FFTW accept it: lengths whose primes are <= 13 run
+ * the mixed-radix kernels, others Bluestein's chirp z-transform, odd R2C/C2R lengths a
+ * direct odd-length kernel.  A length whose work line (n, or the Bluestein length >= 2n-1)
+ * exceeds the 160 KiB of LDS of one workgroup returns DTFFT_HIP_ERROR_UNSUPPORTED_SIZE. */
 int32_t dtfft_hip_fft_create(int8_t fft_rank, int8_t fft_type, int32_t precision, int32_t idist,
                              int32_t odist, int32_t how_many, const int32_t* fft_sizes,
                              const int32_t* inembed, const int32_t* onembed,

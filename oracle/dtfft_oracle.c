@@ -548,7 +548,11 @@ static void orc_fft_line_##SUFFIX(const orc_fft_plan* p, REAL* x, REAL* y, int s
     const int n = p->n;                                                                     \
     REAL* src = x; REAL* dst = y;                                                           \
     long L = 1;                                                                             \
-    REAL dft_re[64], dft_im[64];                                                            \
+    int maxr = 4;                                                                           \
+    for (int f = 0; f < p->nf; f++) if (p->radix[f] > maxr) maxr = p->radix[f];             \
+    /* per-radix buffers: any prime can be a radix (generic O(R^2) DFT) */                  \
+    REAL* rbuf = (REAL*)malloc(sizeof(REAL) * 4 * (size_t)maxr);                            \
+    REAL *dft_re = rbuf, *dft_im = rbuf + maxr, *ar = rbuf + 2 * maxr, *ai = rbuf + 3 * maxr; \
     for (int f = 0; f < p->nf; f++) {                                                       \
         const int R = p->radix[f];                                                          \
         const long nR = n / R, LR = L * R, wstep = n / LR;                                  \
@@ -561,7 +565,6 @@ static void orc_fft_line_##SUFFIX(const orc_fft_plan* p, REAL* x, REAL* y, int s
         }                                                                                   \
         for (long j = 0; j < nR; j++) {                                                     \
             const long k = j % L;                                                           \
-            REAL ar[64], ai[64];                                                            \
             for (int r = 0; r < R; r++) {                                                   \
                 REAL vr = src[2 * (j + r * nR)], vi = src[2 * (j + r * nR) + 1];            \
                 if (r && k) {                                                               \
@@ -608,6 +611,7 @@ static void orc_fft_line_##SUFFIX(const orc_fft_plan* p, REAL* x, REAL* y, int s
         L = LR;                                                                             \
     }                                                                                       \
     if (src != x) memcpy(x, src, sizeof(REAL) * 2 * (size_t)n);                             \
+    free(rbuf);                                                                             \
 }                                                                                            \
                                                                                              \
 /* Batched contiguous C2C: howmany lines of n complex values, idist = n. */                 \

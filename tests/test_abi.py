@@ -67,13 +67,15 @@ def test_argument_validation(D):
                                   C.byref(h)) == 11
     assert L.dtfft_hip_fft_create(1, 2, 0, 8, 8, 1, _i32([8]), _i32([8]), _i32([8]), _i32([11]), 0, None,
                                   C.byref(h)) == 12
-    # plans: R2R with an executor needs kinds; DST-1 of 100 needs a 202-point FFT (prime 101)
+    # plans: R2R with an executor needs kinds
     assert L.dtfft_hip_create_plan_r2r(3, _i32([4, 4, 4]), None, 0, 0, 5, C.byref(h)) == 11
     assert L.dtfft_hip_create_plan_r2r(3, _i32([4, 4, 4]), _i32([5, 5, 2]), 0, 0, 5, C.byref(h)) == 12
-    assert L.dtfft_hip_create_plan_r2r(2, _i32([4, 100]), _i32([5, 7]), 1, 0, 5, C.byref(h)) == 602
-    # executor: a length with a prime factor > 13
-    assert L.dtfft_hip_fft_create(1, 0, 0, 17, 17, 1, _i32([17]), _i32([17]), _i32([17]), None, 0, None,
-                                  C.byref(h)) == 602
+    # executor: a prime length whose Bluestein work line (>= 2n - 1 points) exceeds the LDS of
+    # one workgroup (FP64: 10240 points); lengths with a prime factor > 13 are otherwise taken
+    assert L.dtfft_hip_fft_create(1, 0, 1, 6007, 6007, 1, _i32([6007]), _i32([6007]), _i32([6007]), None, 0,
+                                  None, C.byref(h)) == 602
+    # DST-1 of 5200 needs a 10402-point FFT (2 x 7 x 743), beyond that in FP64 as well
+    assert L.dtfft_hip_create_plan_r2r(2, _i32([4, 5200]), _i32([5, 7]), 1, 0, 5, C.byref(h)) == 602
     # destroying nothing
     assert L.dtfft_hip_destroy(C.byref(C.c_void_p())) == 1
 

@@ -50,12 +50,12 @@ def test_metric_512_fp32_forward_against_fp64_oracle(fused):
     b = D.DeviceArray(x.nbytes)
     c = D.DeviceArray(x.nbytes)
     plan.execute(a, b, D.Execute.FORWARD)
+    # the plain backward (execute_generic) overwrites its input, as the reference's does
+    got = b.download((n, n, n), np.complex64)
     plan.execute(b, c, D.Execute.BACKWARD)
     back = c.download(x.size, np.complex64)
     rt = rel((back / np.float32(n ** 3)).astype(np.complex128), x.astype(np.complex128))
-    del back, c
-    got = b.download((n, n, n), np.complex64)
-    del a, b
+    del back, a, b, c
     want = OP.execute_generic(x.reshape(n, n, n).astype(np.complex128), True)
     err = rel(got.astype(np.complex128), want)
     print(f"forward rel L2 {err:.3e}, round trip {rt:.3e}", flush=True)

@@ -41,12 +41,18 @@ def test_r2c_c2r_golden(n):
 
 
 @pytest.mark.parametrize("n", [48, 60, 64, 84, 96, 105, 128, 160, 192, 210, 256, 320, 384,
-                               512, 1024, 2048, 11, 13, 121, 169])
+                               512, 1024, 2048, 11, 13, 121, 169, 37, 101, 1022, 4099])
 def test_against_naive_dft(n):
     rng = np.random.default_rng(n)
     x = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+    # a prime radix p is an O(p^2) sum in the oracle: its rounding grows with sqrt(p)
+    m = n
+    for p in (2, 3, 5, 7, 11, 13):
+        while m % p == 0:
+            m //= p
+    tol = 2e-15 if m == 1 else 2e-15 * max(1.0, np.sqrt(m) / 8)
     for sign in (O.FFT_FORWARD, O.FFT_BACKWARD):
-        assert rel(O.fft_c2c(x, sign), O.dft_naive(x, sign)) < 2e-15
+        assert rel(O.fft_c2c(x, sign), O.dft_naive(x, sign)) < tol
 
 
 @pytest.mark.parametrize("n", [1, 7, 64, 105, 512])
@@ -135,3 +141,22 @@ def test_plan_r2c_2d_against_rfft2(shape):
     assert rel(z, np.fft.rfft2(x).T) < 1e-14
     back = OP.execute_2d_c2r(z, shape[1]) / x.size
     assert rel(back, x) < 1e-14
+
+
+@pytest.mark.parametrize("n", [1, 3, 7, 35, 74, 1023])
+def test_real_odd_and_prime_lengths(n):
+    """R2C / C2R of odd lengths and of halves with a prime factor above 13, against the
+    long-double DFT of the real line and of the Hermitian extension."""
+    rng = np.random.default_rng(n + 3)
+    r = rng.standard_normal(n)
+    full = O.dft_naive(r.astype(np.complex128))
+    assert rel(O.fft_r2c(r), full[: n // 2 + 1]) < 2e-15
+    h = rng.standard_normal(n // 2 + 1) + 1j * rng.standard_normal(n // 2 + 1)
+    h[0] = h[0].real
+    if n % 2 == 0:
+        h[-1] = h[-1].real
+    ext = np.concatenate([h, np.conj(h[1:(n + 1) // 2][::-1])])
+    assert ext.size == n
+    want = O.dft_naive(ext, O.FFT_BACKWARD)
+    assert np.abs(want.imag).max() < 1e-12 * max(1.0, np.abs(want).max())
+    assert rel(O.fft_c2r(h, n), want.real) < 2e-15
