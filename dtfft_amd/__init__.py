@@ -297,6 +297,12 @@ class _Plan:
         call("dtfft_hip_execute", self.h, C.c_void_p(_ptr(inp)), C.c_void_p(_ptr(out)), int(execute_type),
              C.c_void_p(_ptr(aux)))
 
+    def execute_no_alloc(self, inp, out, execute_type: Execute) -> None:
+        """execute() with the library's fixed per-device scratch as aux: allocates nothing
+        (dtfft_hip_execute_no_alloc); calls that overlap on one GPU race."""
+        call("dtfft_hip_execute_no_alloc", self.h, C.c_void_p(_ptr(inp)), C.c_void_p(_ptr(out)),
+             int(execute_type))
+
     def execute_cycle_graph(self, inp, out, in2, aux=None) -> None:
         """Forward (inp -> out) then backward (out -> in2), captured once as a hipGraph and
         replayed (dtfft_hip_execute_cycle_graph)."""

@@ -60,6 +60,7 @@ SIGNATURES = {
     "dtfft_hip_create_plan_r2c": (_i32, [_i8, _pi32, _i32, _i32, _i32, _pvp]),
     "dtfft_hip_create_plan_r2r": (_i32, [_i8, _pi32, _pi32, _i32, _i32, _i32, _pvp]),
     "dtfft_hip_execute": (_i32, [_vp, _vp, _vp, _i32, _vp]),
+    "dtfft_hip_execute_no_alloc": (_i32, [_vp, _vp, _vp, _i32]),
     "dtfft_hip_transpose": (_i32, [_vp, _vp, _vp, _i32, _vp]),
     "dtfft_hip_destroy": (_i32, [_pvp]),
     "dtfft_hip_execute_cycle_graph": (_i32, [_vp, _vp, _vp, _vp, _vp]),

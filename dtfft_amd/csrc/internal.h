@@ -189,6 +189,13 @@ int32_t plan_c2r_cols_stage(int n, bool dbl, int64_t nl, int64_t nb, int64_t in_
                             int64_t out_bs, fft_stage* st);
 int32_t get_rtwiddles(int n, bool dbl, std::shared_ptr<twiddles>* out);
 
+// ---- allocation-free execute (noalloc.hip) ----------------------------------------
+// Bytes of the per-device scratch array that dtfft_hip_execute_no_alloc uses as aux.
+constexpr size_t kNoAllocAuxBytes = DTFFT_HIP_NO_ALLOC_AUX_BYTES;
+constexpr int kNoAllocMaxDevices = 64;
+// Device address of the calling thread's current device's scratch array.
+int32_t no_alloc_scratch(void** out);
+
 // ---- exact data movement (kernel plugin) ----------------------------------------
 struct box_desc {  // mirrors box_desc in device/transpose.hip
     int64_t e0, e1, e2;

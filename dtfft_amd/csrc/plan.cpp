@@ -453,6 +453,26 @@ int32_t dtfft_hip_execute(dtfft_hip_plan_t plan, void* in, void* out, int32_t ex
     return exec_steps(p, in, out, fwd, aux, r);
 }
 
+int32_t dtfft_hip_execute_no_alloc(dtfft_hip_plan_t plan, void* in, void* out, int32_t execute_type) {
+    auto* p = (plan_t*)plan;
+    if (!p) return DTFFT_HIP_ERROR_PLAN_NOT_CREATED;
+    if (execute_type != DTFFT_HIP_EXECUTE_FORWARD && execute_type != DTFFT_HIP_EXECUTE_BACKWARD)
+        return DTFFT_HIP_ERROR_INVALID_EXECUTE_TYPE;
+    if (!in || !out) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    if (compile_only()) return DTFFT_HIP_ERROR_NO_DEVICE;
+    if (p->transpose_only && in == out) return DTFFT_HIP_ERROR_INPLACE_TRANSPOSE;
+    const size_t need = p->alloc_elems * p->elem_size;
+    if (need > kNoAllocAuxBytes) {
+        set_last_error("plan needs " + std::to_string(need) + " aux bytes, the no-alloc scratch holds " +
+                       std::to_string(kNoAllocAuxBytes));
+        return DTFFT_HIP_ERROR_SCRATCH_TOO_SMALL;
+    }
+    void* aux = nullptr;
+    TRY(no_alloc_scratch(&aux));
+    steps r;
+    return exec_steps(p, in, out, execute_type == DTFFT_HIP_EXECUTE_FORWARD, aux, r);
+}
+
 int32_t dtfft_hip_execute_cycle_graph(dtfft_hip_plan_t plan, void* in, void* out, void* aux, void* in2) {
     auto* p = (plan_t*)plan;
     if (!p) return DTFFT_HIP_ERROR_PLAN_NOT_CREATED;

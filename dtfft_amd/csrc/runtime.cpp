@@ -315,6 +315,7 @@ const char* dtfft_hip_get_error_string(int32_t code) {
         case DTFFT_HIP_ERROR_INDEX_OVERFLOW: return "Local size exceeds 2^31-1 elements";
         case DTFFT_HIP_ERROR_NO_DEVICE: return "No gfx950 device available";
         case DTFFT_HIP_ERROR_INVALID_ARGUMENT: return "Invalid argument";
+        case DTFFT_HIP_ERROR_SCRATCH_TOO_SMALL: return "Aux buffer larger than the no-alloc scratch";
         default: return "Unknown error";
     }
 }

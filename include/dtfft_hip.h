@@ -71,6 +71,7 @@ extern "C" {
 #define DTFFT_HIP_ERROR_INDEX_OVERFLOW           605  /* > 2^31-1 elements per buffer */
 #define DTFFT_HIP_ERROR_NO_DEVICE                606  /* no gfx950 device visible     */
 #define DTFFT_HIP_ERROR_INVALID_ARGUMENT         607
+#define DTFFT_HIP_ERROR_SCRATCH_TOO_SMALL        608  /* aux > DTFFT_HIP_NO_ALLOC_AUX_BYTES */
 
 /* ---- enums (dtFFT values) ---------------------------------------------------------- */
 /* kernel_type_t, src/dtfft_abstract_kernel.F90:59-98 */
@@ -232,6 +233,15 @@ int32_t dtfft_hip_destroy(dtfft_hip_plan_t* plan);
    pointers change.  Ordered on the plan's stream like two execute calls; the backward may
    overwrite out, as execute BACKWARD may. */
 int32_t dtfft_hip_execute_cycle_graph(dtfft_hip_plan_t plan, void* in, void* out, void* aux, void* in2);
+/* No counterpart in the reference: dtfft_hip_execute without an aux argument and without any
+   allocation, for callers that cannot allocate (code under stream capture, tight loops,
+   harnesses that forbid hipMalloc).  The aux buffer of the sequence is a fixed per-device
+   array of DTFFT_HIP_NO_ALLOC_AUX_BYTES inside the library (each GPU the library runs on holds
+   it until the process ends); a plan whose aux (dtfft_hip_get_aux_bytes) is larger returns
+   DTFFT_HIP_ERROR_SCRATCH_TOO_SMALL before any launch.  All calls on one GPU share that
+   array: calls that overlap on one GPU (other streams, other threads) race. */
+#define DTFFT_HIP_NO_ALLOC_AUX_BYTES ((size_t)128 << 20)
+int32_t dtfft_hip_execute_no_alloc(dtfft_hip_plan_t plan, void* in, void* out, int32_t execute_type);
 /* starts/counts in reverse order, as dtfft_get_local_sizes; alloc_size in elements. */
 int32_t dtfft_hip_get_local_sizes(dtfft_hip_plan_t plan, int32_t* in_starts, int32_t* in_counts,
                                   int32_t* out_starts, int32_t* out_counts, size_t* alloc_size);
