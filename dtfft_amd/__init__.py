@@ -431,6 +431,39 @@ class ExchangeShare:
             self.fft.execute(tmp, B, sign)
 
 
+class PencilTranspose:
+    """One rank's transpose of a P > 1 plan on the GPU: the pack and unpack kernels that
+    reshape_handle_generic%create picks, with their neighbour data
+    (src/dtfft_reshape_handle_generic.F90:174-693, restated in pencil.TransposeShare), run in
+    the pack -> exchange -> unpack order of its execute (:695-759).  Every transpose type,
+    both directions (the 3D backward ones as the two-step PERMUTE_BACKWARD_START /
+    PERMUTE_BACKWARD_END pair), uneven splits, modes "mpi" / "pipelined" / "fused" /
+    "unpack".  The exchange is the caller's: on one GPU, a device copy of each peer chunk
+    (send_counts / send_displs to recv_displs of the peer)."""
+
+    def __init__(self, transpose_type: int, dims, comm_size: int, me: int, base_storage: int, mode: str = "mpi",
+                 effort: Effort = Effort.ESTIMATE, fixed=(None, 0)):
+        from . import pencil
+        self.share = sh = pencil.TransposeShare(transpose_type, dims, comm_size, me, mode, fixed)
+        self.es = int(base_storage)
+        self.pack = Kernel(sh.pack_kernel, sh.in_counts, self.es, sh.send_rows, effort)
+        self.unpack = Kernel(sh.unpack_kernel, sh.out_counts, self.es, sh.recv_rows, effort)
+
+    @property
+    def send_bytes(self) -> int:
+        return sum(self.share.send_counts) * self.es
+
+    @property
+    def recv_bytes(self) -> int:
+        return sum(self.share.recv_counts) * self.es
+
+    def pack_into(self, inp, send, stream: int | None = None) -> None:
+        self.share.run_pack(lambda i: self.pack.execute(inp, send, neighbor=i, stream=stream))
+
+    def unpack_from(self, recv, out, stream: int | None = None) -> None:
+        self.share.run_unpack(lambda i: self.unpack.execute(recv, out, neighbor=i, stream=stream))
+
+
 class Cycle:
     """Batched fwd+bwd cycle over many 3D C2C grids (dims per grid in Fortran order)."""
 

@@ -56,3 +56,221 @@ def forward_exchange(a_global: int, b_global: int, c_local: int, comm_size: int,
         rcount.append(m)
         rdispl += m
     return send, recv, scount, rcount
+
+
+# ---------------------------------------------------------------------------------------
+# Every transpose of a P > 1 plan, both directions, uneven splits
+# ---------------------------------------------------------------------------------------
+# dtfft_transpose_t values (include/dtfft_config.h.in:35-43)
+X_TO_Y, Y_TO_X, Y_TO_Z, Z_TO_Y, X_TO_Z, Z_TO_X = 1, -1, 2, -2, 3, -3
+FORWARD_TYPES = (X_TO_Y, Y_TO_Z, Z_TO_X)
+
+# kernel_type_t values (src/dtfft_abstract_kernel.F90:59-98)
+KERNEL_PACK, KERNEL_UNPACK = 1, 3
+KERNEL_UNPACK_PIPELINED, KERNEL_PACK_PIPELINED = 5, 6
+KERNEL_PERMUTE_FORWARD, KERNEL_PERMUTE_BACKWARD = 7, 8
+KERNEL_PERMUTE_BACKWARD_START, KERNEL_PERMUTE_BACKWARD_END = 9, 10
+KERNEL_PERMUTE_BACKWARD_END_PIPELINED = 11
+KERNEL_PACK_FORWARD, KERNEL_PACK_BACKWARD = 12, 13
+KERNEL_UNPACK_FORWARD, KERNEL_UNPACK_FORWARD_PIPELINED = 15, 16
+KERNEL_UNPACK_BACKWARD, KERNEL_UNPACK_BACKWARD_PIPELINED = 17, 18
+
+# How the exchange is driven (the backend families of src/dtfft_reshape_handle_generic.F90:
+# 401-413, 582-600): "mpi" packs the whole block, exchanges, then unpacks every peer;
+# "pipelined" unpacks peer by peer as chunks arrive; "fused" packs peer by peer straight
+# into the send chunks (get_fused) and unpacks peer by peer; "unpack" is the host's
+# transpose mode UNPACK (plain per-peer copies out, the permutation on the receive side).
+MODES = ("mpi", "pipelined", "fused", "unpack")
+
+
+def sub_pencils(transpose_type: int, dims, comm_size: int, fixed=(None, 0)):
+    """Send and receive pencils of every rank of the 1D sub-communicator a transpose runs in.
+
+    ``dims`` are the global dims (X first).  Pencils follow get_permutations and
+    create_pencils_and_comm (src/dtfft_transpose_plan.F90:1045-1131): X = [X, Y, Z],
+    Y = [Y, Z, X], Z = [Z, X, Y] (2D: X = [X, Y], Y = [Y, X]).  X<->Y exchanges inside the
+    communicator that splits Y of the X-pencil, Y<->Z inside the one that splits Z, X<->Z
+    (Z-slab) inside the one that splits Z while Y is whole.  The axis that the transpose
+    leaves alone keeps this rank's (count, start) = ``fixed`` (count None: the whole axis).
+    Returns a list over ranks of (in_counts, in_starts, out_counts, out_starts), each in the
+    pencil's own axis order."""
+    nd = len(dims)
+    up = transpose_type > 0  # X->Y, Y->Z, X->Z run from the lower pencil to the higher one
+    tt = abs(transpose_type)
+    res = []
+    for r in range(comm_size):
+        if nd == 2:
+            nx, ny = dims
+            ys, yn = local_size(ny, comm_size, r)
+            xs, xn = local_size(nx, comm_size, r)
+            xp = ((nx, yn), (0, ys))
+            yp = ((ny, xn), (0, xs))
+            a, b = (xp, yp) if tt == X_TO_Y else (None, None)
+        else:
+            nx, ny, nz = dims
+            if tt == X_TO_Y:
+                fc = nz if fixed[0] is None else fixed[0]
+                ys, yn = local_size(ny, comm_size, r)
+                xs, xn = local_size(nx, comm_size, r)
+                a = ((nx, yn, fc), (0, ys, fixed[1]))
+                b = ((ny, fc, xn), (0, fixed[1], xs))
+            elif tt == Y_TO_Z:
+                fc = nx if fixed[0] is None else fixed[0]
+                zs, zn = local_size(nz, comm_size, r)
+                ys, yn = local_size(ny, comm_size, r)
+                a = ((ny, zn, fc), (0, zs, fixed[1]))
+                b = ((nz, fc, yn), (0, fixed[1], ys))
+            else:  # X_TO_Z, Z-slab: Y whole in the X-pencil, X whole in the Z-pencil
+                zs, zn = local_size(nz, comm_size, r)
+                ys, yn = local_size(ny, comm_size, r)
+                a = ((nx, ny, zn), (0, 0, zs))
+                b = ((nz, nx, yn), (0, 0, ys))
+        if nd == 2 and tt != X_TO_Y:
+            raise ValueError("2D plans have X<->Y transposes only")
+        send, recv = (a, b) if up else (b, a)
+        res.append((send[0], send[1], recv[0], recv[1]))
+    return res
+
+
+def transpose_neighbor_data(transpose_type: int, ndims: int, pencils, me: int):
+    """neighbor_data of the pack and unpack kernels of one rank: the transpose branch of
+    reshape_handle_generic%create, send side src/dtfft_reshape_handle_generic.F90:291-423 and
+    receive side :485-616.  ``pencils`` is sub_pencils' list (every rank's send and receive
+    pencil, as the handles' MPI_Allgather collects them, :136-145).
+
+    Returns (send_rows, recv_rows, send_counts, recv_counts): rows (nxx, nyy, nzz, din, dout)
+    per peer in peer order, counts in elements (the MPI_Alltoallv counts)."""
+    P = len(pencils)
+    in_sizes = [p[0] for p in pencils]
+    in_starts = [p[1] for p in pencils]
+    out_sizes = [p[2] for p in pencils]
+    out_starts = [p[3] for p in pencils]
+
+    def send_ln_ls(src: int, i: int):
+        """in%ln(:, i), in%ls(:, i) as computed on rank `src` for peer i (:295-334)."""
+        if ndims == 2:
+            return (out_sizes[i][1], in_sizes[src][1]), (out_starts[i][1], in_starts[src][1])
+        if transpose_type == X_TO_Z:
+            return ((in_sizes[src][0], out_sizes[i][2], in_sizes[src][2]),
+                    (in_starts[src][0], out_starts[i][2], in_starts[src][2]))
+        if transpose_type in (Z_TO_X, X_TO_Y, Y_TO_Z):
+            return ((out_sizes[i][2], in_sizes[src][1], in_sizes[src][2]),
+                    (out_starts[i][2], in_starts[src][1], in_starts[src][2]))
+        return ((out_sizes[i][1], in_sizes[src][1], in_sizes[src][2]),
+                (out_starts[i][1], in_starts[src][1], in_starts[src][2]))
+
+    def prod(v):
+        n = 1
+        for x in v:
+            n *= int(x)
+        return n
+
+    send, scount = [], []
+    sdispl = 0
+    for i in range(P):
+        ln, ls = send_ln_ls(me, i)
+        din = ln[0] * ls[1] if transpose_type == X_TO_Z else ls[0]
+        send.append((ln[0], ln[1], ln[2] if ndims == 3 else 1, din, sdispl))
+        n = prod(ln)
+        scount.append(n)
+        sdispl += n
+
+    recv, rcount = [], []
+    rdispl = 0
+    for i in range(P):
+        recvsize = prod(send_ln_ls(i, me)[0])  # what rank i sends here (:487-488)
+        ln, ls = (0, 0, 0), (0, 0, 0)
+        if recvsize > 0:
+            if ndims == 2:
+                ln = (in_sizes[i][1], out_sizes[me][1], 1)
+                ls = (in_starts[i][1], out_starts[me][1], 0)
+            elif transpose_type == X_TO_Z:
+                ln = (in_sizes[i][2], out_sizes[me][1], out_sizes[me][2])
+                ls = (in_starts[i][2], out_starts[me][1], out_starts[me][2])
+            elif transpose_type == Z_TO_X:
+                ln = (out_sizes[me][0], in_sizes[i][2], out_sizes[me][2])
+                ls = (out_starts[me][0], in_starts[i][2], out_starts[me][2])
+            elif transpose_type in (X_TO_Y, Y_TO_Z):
+                ln = (in_sizes[i][1], out_sizes[me][1], out_sizes[me][2])
+                ls = (in_starts[i][1], in_starts[me][1], out_starts[me][2])
+            else:
+                ln = (in_sizes[i][2], out_sizes[me][1], out_sizes[me][2])
+                ls = (in_starts[i][2], in_starts[me][1], out_starts[me][2])
+        dout = ln[0] * ls[1] if transpose_type == Z_TO_X else ls[0]
+        recv.append((ln[0], ln[1], ln[2] if ndims == 3 else 1, rdispl, dout))
+        rcount.append(recvsize)
+        rdispl += recvsize
+    return send, recv, scount, rcount
+
+
+def transpose_kernel_types(transpose_type: int, ndims: int, mode: str):
+    """(pack kernel, unpack kernel, two_step) of a P > 1 transpose, as
+    src/dtfft_reshape_handle_generic.F90:236-245 (permute direction), :401-413 (two-step
+    backward, transpose mode UNPACK, get_fused) and :582-600 (unpack side) select them.
+    dtFFT on CUDA ignores the UNPACK transpose mode (:211); here it is kept for the host
+    restatement and the GPU kernels alike."""
+    if mode not in MODES:
+        raise ValueError(f"mode must be one of {MODES}")
+    pack = KERNEL_PERMUTE_FORWARD if transpose_type in FORWARD_TYPES else KERNEL_PERMUTE_BACKWARD
+    fused = mode == "fused"
+    pipelined = mode in ("pipelined", "fused")
+    two_step = transpose_type in (Y_TO_X, Z_TO_Y) and ndims == 3 and not fused
+    if two_step:
+        pack = KERNEL_PERMUTE_BACKWARD_START
+    if mode == "unpack":
+        pack = KERNEL_PACK
+    if fused:
+        pack = {KERNEL_PACK: KERNEL_PACK_PIPELINED, KERNEL_PERMUTE_FORWARD: KERNEL_PACK_FORWARD,
+                KERNEL_PERMUTE_BACKWARD: KERNEL_PACK_BACKWARD}.get(pack, pack)
+    unpack = KERNEL_UNPACK_PIPELINED if pipelined else KERNEL_UNPACK
+    if two_step:
+        unpack = KERNEL_PERMUTE_BACKWARD_END_PIPELINED if pipelined else KERNEL_PERMUTE_BACKWARD_END
+    if mode == "unpack":
+        if transpose_type in FORWARD_TYPES:
+            unpack = KERNEL_UNPACK_FORWARD_PIPELINED if pipelined else KERNEL_UNPACK_FORWARD
+        else:
+            unpack = KERNEL_UNPACK_BACKWARD_PIPELINED if pipelined else KERNEL_UNPACK_BACKWARD
+    return pack, unpack, two_step
+
+
+class TransposeShare:
+    """One rank's P > 1 transpose: pencils, neighbour data, kernel types and the
+    pack -> exchange -> unpack order of reshape_handle_generic%execute
+    (src/dtfft_reshape_handle_generic.F90:695-759).  Host logic only: ``run`` drives the
+    caller's kernels (host or GPU) through three callables."""
+
+    def __init__(self, transpose_type: int, dims, comm_size: int, me: int, mode: str = "mpi", fixed=(None, 0)):
+        self.transpose_type, self.dims, self.P, self.me, self.mode = transpose_type, tuple(dims), comm_size, me, mode
+        self.ndims = len(dims)
+        self.pencils = sub_pencils(transpose_type, dims, comm_size, fixed)
+        self.in_counts, self.in_starts, self.out_counts, self.out_starts = self.pencils[me]
+        (self.send_rows, self.recv_rows, self.send_counts,
+         self.recv_counts) = transpose_neighbor_data(transpose_type, self.ndims, self.pencils, me)
+        self.pack_kernel, self.unpack_kernel, self.two_step = transpose_kernel_types(transpose_type, self.ndims,
+                                                                                     mode)
+        self.send_displs = [r[4] for r in self.send_rows]
+        self.recv_displs = [r[3] for r in self.recv_rows]
+
+    @property
+    def per_peer_pack(self) -> bool:
+        return self.pack_kernel in (KERNEL_PACK_FORWARD, KERNEL_PACK_BACKWARD, KERNEL_PACK_PIPELINED)
+
+    @property
+    def per_peer_unpack(self) -> bool:
+        return self.mode in ("pipelined", "fused")
+
+    def run_pack(self, pack):
+        """pack(neighbor) for each peer (1-based) when the pack kernel is per peer, else pack(0)
+        once (a whole-block permute, or PACK over every peer)."""
+        if self.per_peer_pack:
+            for i in range(1, self.P + 1):
+                pack(i)
+        else:
+            pack(0)
+
+    def run_unpack(self, unpack):
+        if self.per_peer_unpack:
+            for i in range(1, self.P + 1):
+                unpack(i)
+        else:
+            unpack(0)
