@@ -885,3 +885,212 @@ fft_cols_dma1_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
         __syncthreads();  // the staged rows are read before the next tile's loads land
     }
 }
+
+// ---------------------------------------------------------------------------
+// Fused Z->X transpose + C2R (the backward of a fused R2C plan) through one LDS tile filled
+// by direct global->LDS loads, one tile per block (two blocks per CU).  A tile is LPB
+// adjacent lines (one 128-byte piece per input row) of the M + 1 input rows X_0 .. X_M.
+//   1. load: rows 0 .. M in the load layout of cols_dma (groups of 8 rows, 1152 bytes apart);
+//   2. untangle in place: one thread owns the pair of rows (k, M - k) of a line and turns
+//      X_k, X_{M-k} into conj(Z_k), conj(Z_{M-k}) (Z_k = 2E_k + i 2O_k as fft_c2r_kernel),
+//      so every input element is read from HBM once;
+//   3. the M-point passes of cols_dma in place, the last one writing the line-major staging
+//      layout, and the rows of M complex (2M reals) stored whole, conjugated.
+// TWLDS: the FFT twiddle table in LDS (one block per CU at M = 512) or read from global
+// memory (the tile alone fits two blocks per CU); the untangling twiddles are read from
+// global memory.  The host uses this kernel only when nl % LPB == 0, in_ls == 1, out_ns == 1,
+// every input row piece is 16-byte aligned and the LDS fits.
+// ---------------------------------------------------------------------------
+namespace dtfft {
+template <typename T, class S, int TPL, int LPB, int PAD>
+struct c2r_dma {
+    typedef cols_dma<T, S, TPL, LPB, PAD, false> C;
+    typedef typename tw_of<T>::type tw_t;
+    static constexpr int ES = C::ES;
+    static constexpr int M = S::N;
+    static constexpr int ROWS = M + 1;
+    static constexpr int NGL = (ROWS + 7) / 8;
+    // the last group holds ROWS - 8 (NGL - 1) rows and needs no padding after them
+    static constexpr int LOADB = (NGL - 1) * C::GROUPB + (ROWS - 8 * (NGL - 1)) * 128;
+    static constexpr int BUFE = (LOADB > C::STAGEB ? LOADB : C::STAGEB) / ES;
+    static constexpr int RTWN = M / 2 + 1;  // untangling twiddles kept: W^k, k <= M/2
+
+    // this wave's share of the M + 1 rows of the tile at ibase
+    static __device__ __forceinline__ void load(const cx<T>* in, cx<T>* buf, const lines_desc& d, i64 ibase) {
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const int row = lane >> 3, chunk = lane & 7;
+        for (int q = wave; q < NGL; q += C::NW) {
+            if (q * 8 + row < ROWS) {
+                const char* g = (const char*)(in + ibase + (i64)(q * 8 + row) * d.in_ns) + chunk * 16;
+                DTFFT_LOAD_LDS16(g, (char*)buf + q * C::GROUPB);
+            }
+        }
+    }
+
+    // conj(Z_k) from X_k (a) and X_{M-k} (b): 2E_k = a + conj b, 2O_k = (a - conj b) conj(W^k)
+    static __device__ __forceinline__ cx<T> untangle(cx<T> a, cx<T> b, const tw_t w) {
+        b = conj(b);
+        const cx<T> e2 = a + b;
+        const cx<T> o2 = twmul(a - b, conj_tw(w));
+        return conj(mk(e2.x - o2.y, e2.y + o2.x));
+    }
+
+    // -conj(w): W^{M-k} = exp(-i pi (M - k) / M) = -conj(W^k)
+    static __device__ __forceinline__ twf mirror(twf w) { w.rh = -w.rh; w.rl = -w.rl; return w; }
+    static __device__ __forceinline__ cx<double> mirror(cx<double> w) { w.x = -w.x; return w; }
+
+    // rtw holds W^k for k <= M / 2 (the other half follows by the mirror)
+    static __device__ __forceinline__ void untangle_tile(cx<T>* buf, const tw_t* rtw) {
+        const int line = threadIdx.x % LPB;
+        for (int k = threadIdx.x / LPB; k <= M / 2; k += C::NT / LPB) {
+            cx<T> a = buf[C::a_off(k, line)];
+            cx<T> b = buf[C::a_off(M - k, line)];
+            if (k == 0) { a.y = (T)0; b.y = (T)0; }  // Im X_0, Im X_M do not contribute
+            const tw_t w = rtw[k];
+            const cx<T> zk = untangle(a, b, w);
+            if (k > 0 && 2 * k < M) buf[C::a_off(M - k, line)] = untangle(b, a, mirror(w));
+            buf[C::a_off(k, line)] = zk;
+        }
+    }
+
+    // The untangling fused into the first radix pass.  Butterfly b of pass 0 reads rows
+    // b + r NB0; the mirror of row b + r NB0 is row (NB0 - b) + (R0 - 1 - r) NB0, a row of
+    // butterfly NB0 - b (butterflies 0 and NB0 / 2 mirror onto themselves, row 0 onto the
+    // extra row M).  With two butterflies per thread, thread j takes b = j and b = NB0 - j
+    // (thread 0: b = 0 and NB0 / 2), so every mirror pair is in one thread's registers and
+    // the untangled values go straight into the butterflies: one LDS sweep less per tile.
+    static constexpr int R0 = S::radix(0);
+    static constexpr int NB0 = M / R0;
+    static constexpr bool FUSE0 = NB0 % 2 == 0 && NB0 == 2 * TPL;
+
+    static __device__ __forceinline__ tw_t wk(const tw_t* rtw, int n) {
+        return 2 * n <= M ? rtw[n] : mirror(rtw[M - n]);
+    }
+
+    static __device__ __forceinline__ void pass0_untangle(cx<T>* buf, const tw_t* rtw) {
+        const int line = threadIdx.x % LPB, j = threadIdx.x / LPB;
+        const int b0 = j, b1 = j == 0 ? NB0 / 2 : NB0 - j;
+        cx<T> x0[R0], x1[R0];
+#pragma unroll
+        for (int r = 0; r < R0; r++) {
+            x0[r] = buf[C::a_off(b0 + r * NB0, line)];
+            x1[r] = buf[C::a_off(b1 + r * NB0, line)];
+        }
+        const cx<T> xm = buf[C::a_off(M, line)];  // X_M, the mirror of row 0 (thread 0)
+        DTFFT_LDS_BARRIER();  // every read of the tile precedes the in-place writes
+        cx<T> z0[R0], z1[R0];
+#pragma unroll
+        for (int r = 0; r < R0; r++) {
+            cx<T> a = x0[r];
+            cx<T> m = j == 0 ? (r == 0 ? xm : x0[(R0 - r) % R0]) : x1[R0 - 1 - r];
+            if (j == 0 && r == 0) { a.y = (T)0; m.y = (T)0; }  // Im X_0, Im X_M do not contribute
+            z0[r] = untangle(a, m, wk(rtw, b0 + r * NB0));
+            const cx<T> m1 = j == 0 ? x1[R0 - 1 - r] : x0[R0 - 1 - r];
+            z1[r] = untangle(x1[r], m1, wk(rtw, b1 + r * NB0));
+        }
+        bfly<T, R0>::run(z0);
+        bfly<T, R0>::run(z1);
+#pragma unroll
+        for (int s = 0; s < R0; s++) {
+            if constexpr (S::NP == 1) {  // pass 0 is the last: the line-major staging layout
+                buf[line * C::LD + b0 * R0 + s] = z0[s];
+                buf[line * C::LD + b1 * R0 + s] = z1[s];
+            } else {
+                buf[C::a_off(b0 * R0 + s, line)] = z0[s];
+                buf[C::a_off(b1 * R0 + s, line)] = z1[s];
+            }
+        }
+        DTFFT_LDS_BARRIER();
+    }
+
+    // untangling and the M-point passes on a loaded tile; the transformed lines are left in
+    // the staging layout
+    static __device__ __forceinline__ void transform(cx<T>* buf, const tw_t* twl, const tw_t* rtw) {
+        if constexpr (FUSE0) {
+            pass0_untangle(buf, rtw);
+            C::template passes<1>(buf, twl, false);
+        } else {
+            untangle_tile(buf, rtw);
+            DTFFT_LDS_BARRIER();
+            C::template passes<0>(buf, twl, false);
+        }
+    }
+};
+}  // namespace dtfft
+
+template <typename T, class S, int TPL, int LPB, int PAD, bool TWLDS>
+__global__ void __launch_bounds__(TPL * LPB)
+fft_c2r_dma_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
+                   const typename tw_of<T>::type* __restrict__ tw,
+                   const typename tw_of<T>::type* __restrict__ rtw, dtfft::lines_desc d) {
+    typedef dtfft::c2r_dma<T, S, TPL, LPB, PAD> K;
+    typedef typename K::C C;
+    typedef typename tw_of<T>::type tw_t;
+    __shared__ cx<T> buf[K::BUFE];
+    const tw_t* twp = tw;
+    if constexpr (TWLDS) {
+        __shared__ tw_t twl[C::TWN];
+        for (int i = threadIdx.x; i < S::TW_SIZE; i += C::NT) twl[i] = tw[i];
+        twp = twl;
+    }
+    for (i64 t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
+        i64 ib, ob;
+        C::bases(d, t, ib, ob);
+        K::load(in, buf, d, ib);
+        DTFFT_WAIT_ALL();
+        __syncthreads();  // the tile (and the twiddles) in LDS
+        K::transform(buf, twp, rtw);
+        C::store(buf, out, d, t, true);  // backward = conj(forward(conj z))
+        __syncthreads();  // the staged rows are read before the next tile's loads land
+    }
+}
+
+// Persistent double-buffered variant: one block per CU holds two tile buffers and fills the
+// next tile with direct loads while the current one is untangled, transformed and stored (the
+// order of fft_cols_dma_kernel).  No global load may be issued while a prefetch is in flight
+// (its wait would wait for the prefetch too), so both twiddle tables live in LDS: the FFT
+// table and the untangling half W^k, k <= M/2.
+template <typename T, class S, int TPL, int LPB, int PAD>
+__global__ void __launch_bounds__(TPL * LPB)
+fft_c2r_dma2_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
+                    const typename tw_of<T>::type* __restrict__ tw,
+                    const typename tw_of<T>::type* __restrict__ rtw, dtfft::lines_desc d) {
+    typedef dtfft::c2r_dma<T, S, TPL, LPB, PAD> K;
+    typedef typename K::C C;
+    typedef typename tw_of<T>::type tw_t;
+    __shared__ cx<T> buf_a[K::BUFE];
+    __shared__ cx<T> buf_b[K::BUFE];
+    __shared__ tw_t twl[C::TWN];
+    __shared__ tw_t rtwl[K::RTWN];
+    for (int i = threadIdx.x; i < S::TW_SIZE; i += C::NT) twl[i] = tw[i];
+    for (int i = threadIdx.x; i < K::RTWN; i += C::NT) rtwl[i] = rtw[i];
+    const i64 G = gridDim.x;
+    i64 t = blockIdx.x, ib, ob;
+    if (t < d.ntiles) { C::bases(d, t, ib, ob); K::load(in, buf_a, d, ib); }
+    DTFFT_WAIT_ALL();
+    __syncthreads();  // first tile and the tables in LDS
+    i64 tn = t + G;
+    if (tn < d.ntiles) { C::bases(d, tn, ib, ob); K::load(in, buf_b, d, ib); }
+    for (;;) {
+        if (t >= d.ntiles) break;
+        K::transform(buf_a, twl, rtwl);
+        DTFFT_WAIT_ALL();
+        __syncthreads();  // tile tn is in buf_b
+        C::store(buf_a, out, d, t, true);
+        DTFFT_LDS_BARRIER();  // every wave has read its part of buf_a
+        i64 tnn = tn + G;
+        if (tnn < d.ntiles) { C::bases(d, tnn, ib, ob); K::load(in, buf_a, d, ib); }
+        t = tn;
+        tn = tnn;
+        if (t >= d.ntiles) break;
+        K::transform(buf_b, twl, rtwl);
+        DTFFT_WAIT_ALL();
+        __syncthreads();  // tile tn is in buf_a
+        C::store(buf_b, out, d, t, true);
+        DTFFT_LDS_BARRIER();
+        tnn = tn + G;
+        if (tnn < d.ntiles) { C::bases(d, tnn, ib, ob); K::load(in, buf_b, d, ib); }
+        t = tn;
+        tn = tnn;
+    }
+}

@@ -48,6 +48,14 @@ std::string fft_config::expr() const {
         return s + ">, " + std::to_string(tpl) + ", " + std::to_string(lpb) + ", " + std::to_string(pad) + ", " +
                std::to_string(kind) + (chirp ? ", true>" : ", false>");
     }
+    if (dma && kind == 2) {
+        std::string s = std::string(dma == 1 ? "fft_c2r_dma2_kernel<" : "fft_c2r_dma_kernel<") +
+                        (dbl ? "double" : "float") + ", dtfft::sched<" + std::to_string(n);
+        for (int r : radices) s += ", " + std::to_string(r);
+        s += ">, " + std::to_string(tpl) + ", " + std::to_string(lpb) + ", " + std::to_string(pad);
+        if (dma == 1) return s + ">";
+        return s + (twl ? ", true>" : ", false>");
+    }
     if (dma) {
         std::string s = std::string(dma == 2 ? "fft_cols_dma1_kernel<" : "fft_cols_dma_kernel<") +
                         (dbl ? "double" : "float") + ", dtfft::sched<" + std::to_string(n);
@@ -77,6 +85,19 @@ static const size_t kMaxLds = 160 * 1024;
 // rows, or the line-major staging layout, whichever is larger) and the twiddle table.
 static size_t lds_dma(const fft_config& c) {
     const size_t es = (size_t)elem_bytes(c.dbl);
+    if (c.kind == 2) {
+        // fft_c2r_dma(2)_kernel: buffers of the n + 1 input rows (the last group unpadded) or the
+        // staging layout; dma 1 (double buffer): both twiddle tables in LDS (the FFT table and
+        // the untangling half of n / 2 + 1 entries); dma 2: the FFT table when twl is set
+        const size_t groups = (size_t)((c.n + 1 + 7) / 8);
+        const size_t load = (groups - 1) * 1152 + ((size_t)(c.n + 1) - 8 * (groups - 1)) * 128;
+        const size_t stage = (size_t)c.lpb * (size_t)((c.n + c.pad) | 1) * es;
+        size_t tw = 0, L = 1;
+        for (int r : c.radices) { tw += L * (size_t)(r - 1); L *= (size_t)r; }
+        const size_t twb = std::max<size_t>(1, tw) * 16 + 16;
+        if (c.dma == 1) return 2 * std::max(load, stage) + twb + (size_t)(c.n / 2 + 1) * 16 + 16;
+        return std::max(load, stage) + (c.twl ? twb : 0);
+    }
     const size_t load = (size_t)(c.n / 8) * 1152;
     const size_t stage = (size_t)c.lpb * (size_t)((c.n + c.pad) | 1) * es;
     size_t tw = 0, L = 1;
@@ -84,7 +105,16 @@ static size_t lds_dma(const fft_config& c) {
     return (c.dma == 2 ? 1 : 2) * std::max(load, stage) + std::max<size_t>(1, tw) * 16;
 }
 
+// Whether a fused column-in C2R stage can use fft_c2r_dma_kernel.
+static bool c2r_dma_ok(const fft_config& c, const lines_desc& d) {
+    const int64_t es = elem_bytes(c.dbl);
+    return c.kind == 2 && (c.dma == 1 || c.dma == 2) && c.n > 1 && c.n % 8 == 0 && c.lpb * es == 128 && (c.tpl * c.lpb) % 64 == 0 &&
+           c.tpl * c.lpb <= 1024 && d.in_ls == 1 && d.out_ns == 1 && d.nl % c.lpb == 0 && (d.in_ns * es) % 16 == 0 &&
+           (d.in_bs * es) % 16 == 0 && lds_dma(c) <= kMaxLds;
+}
+
 bool dma_ok(const fft_config& c, const lines_desc& d) {
+    if (c.kind == 2) return c2r_dma_ok(c, d);
     const int64_t es = elem_bytes(c.dbl);
     const bool out_ok = c.out_cols ? (d.out_ls == 1 && (d.out_ns * es) % 16 == 0 && (d.out_bs * es) % 16 == 0)
                                    : d.out_ns == 1;
@@ -391,6 +421,11 @@ int32_t fft_stage::init(const fft_config& c, const lines_desc& desc) {
         }
         fft_config alt = c;
         alt.dma = 0;
+        if (c.kind == 2) {
+            // the fallback for misaligned buffers: the plain fused C2R of the same block shape
+            alt.twl = 0;
+            apply_twl(alt, d, global_config().twl);
+        }
         TRY(rtc_get("fft", alt.expr(), &k_alt));
         grid_alt = grid;
         if (c.dma == 1) {
@@ -428,6 +463,7 @@ int32_t fft_stage::run(const void* in, void* out, int sign, hipStream_t s) const
         void* rtwp = rtw ? rtw->ptr : nullptr;
         lines_desc dd = d;
         void* args[] = {(void*)&in, &out, &twp, &rtwp, &dd};
+        if (cfg.dma && ((uintptr_t)in % 16) != 0) return launch(k_alt, grid_alt, block, s, args);
         return launch(k, grid, block, s, args);
     }
     if (cfg.n == 1) {
@@ -616,7 +652,7 @@ int32_t plan_real_stage(int kind, int n, bool dbl, int64_t how_many, int64_t idi
 }
 
 int32_t plan_c2r_cols_stage(int n, bool dbl, int64_t nl, int64_t nb, int64_t in_bs, int64_t in_ns, int64_t out_ls,
-                            int64_t out_bs, fft_stage* st) {
+                            int64_t out_bs, fft_stage* st, int lds_dma) {
     if (n < 2 || n % 2 != 0) {
         set_last_error("C2R stages need an even length, got " + std::to_string(n));
         return DTFFT_HIP_ERROR_UNSUPPORTED_SIZE;
@@ -639,6 +675,25 @@ int32_t plan_c2r_cols_stage(int n, bool dbl, int64_t nl, int64_t nb, int64_t in_
     // persistent blocks with both twiddle tables in LDS by the configuration's rule (FP32 by
     // default: one 1024-thread block per CU either way)
     apply_twl(c, d, global_config().twl);
+    if (lds_dma != 0) {
+        // LDS-DMA kernels: 128-byte input row pieces loaded straight into LDS, each read once;
+        // two butterflies per thread and pass.  lds_dma 2: the persistent double buffer
+        // (fft_c2r_dma2_kernel, one block per CU); 3: one tile per block (fft_c2r_dma_kernel,
+        // two blocks per CU, the FFT twiddles in LDS only when forced with twl = 2, which leaves
+        // room for one block per CU); 1: the double buffer where it fits, else one tile per block
+        fft_config x = c;
+        x.lpb = 128 / elem_bytes(dbl);
+        x.pad = 1;
+        const int rmax = *std::max_element(x.radices.begin(), x.radices.end());
+        x.tpl = std::max(1, (m / rmax) / 2);
+        const lines_desc dx = make_lines(nl, nb, 1, in_bs, in_ns, out_ls, out_bs, 1, x.lpb);
+        for (int v : {1, 2}) {
+            if ((lds_dma == 2 && v != 1) || (lds_dma == 3 && v != 2)) continue;
+            x.dma = v;
+            x.twl = v == 2 && global_config().twl == 2 ? 1 : 0;
+            if (dma_ok(x, dx)) return st->init(x, dx);
+        }
+    }
     return st->init(c, d);
 }
 

@@ -185,8 +185,9 @@ int32_t get_r2r_table(int64_t n, bool dbl, std::shared_ptr<twiddles>* out);
 // C2R along lines read across (fused transpose): nl lines adjacent in the input, batches
 // in_bs apart, complex element stride in_ns; real output rows of n values viewed as n/2
 // complex, line stride out_ls and batch stride out_bs in those complex units.
+// lds_dma != 0: the LDS-DMA kernel (fft_c2r_dma_kernel) where the stage allows it.
 int32_t plan_c2r_cols_stage(int n, bool dbl, int64_t nl, int64_t nb, int64_t in_bs, int64_t in_ns, int64_t out_ls,
-                            int64_t out_bs, fft_stage* st);
+                            int64_t out_bs, fft_stage* st, int lds_dma = 1);
 int32_t get_rtwiddles(int n, bool dbl, std::shared_ptr<twiddles>* out);
 
 // ---- allocation-free execute (noalloc.hip) ----------------------------------------

@@ -71,7 +71,6 @@ struct plan_t {
     hipStream_t gstream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    hipEvent_t ev_a = nullptr, ev_b = nullptr;
     void* gptr[4] = {nullptr, nullptr, nullptr, nullptr};
 
     plan_t() = default;
@@ -80,8 +79,6 @@ struct plan_t {
         if (gexec) hipGraphExecDestroy(gexec);
         if (graph) hipGraphDestroy(graph);
         if (gstream) hipStreamDestroy(gstream);
-        if (ev_a) hipEventDestroy(ev_a);
-        if (ev_b) hipEventDestroy(ev_b);
         if (aux) hipFree(aux);
         if (work) hipFree(work);
     }
@@ -154,7 +151,7 @@ static int32_t build(plan_t* p) {
         if (p->fused) {
             TRY(fused(p, ny, nc, 1, 0, nc, ny, 0, &p->u_xy));   // X (nc,ny) -> Y (ny,nc)
             if (p->r2c)  // Y (ny,nc) -> X real (nx,ny) + C2R along X
-                TRY(plan_c2r_cols_stage((int)nx, p->dbl, ny, 1, 0, ny, nx / 2, 0, &p->u_yx_c2r));
+                TRY(plan_c2r_cols_stage((int)nx, p->dbl, ny, 1, 0, ny, nx / 2, 0, &p->u_yx_c2r, p->lds_dma));
             else
                 TRY(fused(p, nx, ny, 1, 0, ny, nx, 0, &p->u_yx));   // Y (ny,nx) -> X (nx,ny)
         }
@@ -196,7 +193,8 @@ static int32_t build(plan_t* p) {
             // Z -> X fused with the C2R along X into the real X-pencil rows
             TRY(plan_fft_stage((int)ny, p->dbl, nz * nc, 1, 1, 0, nz * nc, 1, 0, nz * nc, true, E, p->stream,
                                &p->fy_zc, p->lds_dma));
-            TRY(plan_c2r_cols_stage((int)nx, p->dbl, nz, ny, nz * nc, nz, ny * (nx / 2), nx / 2, &p->u_zx_c2r));
+            TRY(plan_c2r_cols_stage((int)nx, p->dbl, nz, ny, nz * nc, nz, ny * (nx / 2), nx / 2, &p->u_zx_c2r,
+                                    p->lds_dma));
         } else {
             TRY(fused(p, nx, nz, ny, nz * nc, nz, nc * ny, nc, &p->u_zx));  // Z -> X + FFT_X
         }
@@ -486,8 +484,6 @@ int32_t dtfft_hip_execute_cycle_graph(dtfft_hip_plan_t plan, void* in, void* out
     if (!p->gexec || !std::equal(ptrs, ptrs + 4, p->gptr)) {
         if (!p->gstream) {
             HIP_TRY(hipStreamCreateWithFlags(&p->gstream, hipStreamNonBlocking));
-            HIP_TRY(hipEventCreateWithFlags(&p->ev_a, hipEventDisableTiming));
-            HIP_TRY(hipEventCreateWithFlags(&p->ev_b, hipEventDisableTiming));
         }
         if (p->gexec) { hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
         if (p->graph) { hipGraphDestroy(p->graph); p->graph = nullptr; }
@@ -507,12 +503,8 @@ int32_t dtfft_hip_execute_cycle_graph(dtfft_hip_plan_t plan, void* in, void* out
         HIP_TRY(hipGraphInstantiate(&p->gexec, p->graph, nullptr, nullptr, 0));
         std::copy(ptrs, ptrs + 4, p->gptr);
     }
-    // the replay runs after the work queued on the plan's stream and before what follows it
-    HIP_TRY(hipEventRecord(p->ev_a, p->stream));
-    HIP_TRY(hipStreamWaitEvent(p->gstream, p->ev_a, 0));
-    HIP_TRY(hipGraphLaunch(p->gexec, p->gstream));
-    HIP_TRY(hipEventRecord(p->ev_b, p->gstream));
-    HIP_TRY(hipStreamWaitEvent(p->stream, p->ev_b, 0));
+    // replayed on the plan's stream: after the work queued there, before what follows it
+    HIP_TRY(hipGraphLaunch(p->gexec, p->stream));
     return DTFFT_HIP_SUCCESS;
 }
 
@@ -651,6 +643,7 @@ int32_t dtfft_hip_report(dtfft_hip_plan_t plan) {
     show("fft_x", p->fx); show("fft_y", p->fy); show("fft_z", p->fz); show("fft_ycol", p->fy_cols);
     show("xy+fft", p->u_xy); show("yz+fft", p->u_yz); show("zx+fft", p->u_zx); show("yx+fft", p->u_yx);
     show("fft_yzc", p->fy_zc); show("zx+c2r", p->u_zx_c2r); show("yx+c2r", p->u_yx_c2r);
+    std::fflush(stdout);
     return DTFFT_HIP_SUCCESS;
 }
 

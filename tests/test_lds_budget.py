@@ -55,3 +55,24 @@ def test_executor_forced_twl_fits(prec):
 @pytest.mark.parametrize("grids", [[[2048, 8, 8]], [[8, 8, 2048]], [[8, 2048, 8], [1024, 8, 8]]])
 def test_cycle_long_axis_fits(grids):
     D.Cycle(grids, D.Precision.SINGLE)
+
+
+@pytest.mark.parametrize("twl", [1, 2])
+@pytest.mark.parametrize("prec,dims,dma", [
+    (D.Precision.SINGLE, (1024, 512, 256), True), (D.Precision.DOUBLE, (1024, 512, 256), True),
+    (D.Precision.SINGLE, (256, 12, 32), True), (D.Precision.SINGLE, (1024, 6, 16), True),
+    (D.Precision.SINGLE, (256, 3, 48), True), (D.Precision.DOUBLE, (512, 6, 16), True),
+    (D.Precision.DOUBLE, (1024, 4, 8), True), (D.Precision.DOUBLE, (512, 5, 24), True),
+    (D.Precision.SINGLE, (256, 64), True), (D.Precision.SINGLE, (1024, 32), True),
+    (D.Precision.DOUBLE, (512, 16), True), (D.Precision.DOUBLE, (1024, 24), True),
+    (D.Precision.SINGLE, (64, 32, 16), False), (D.Precision.SINGLE, (1024, 6, 12), False),
+])
+def test_fused_c2r_lds_dma_kernel_choice(prec, dims, dma, twl, capfd):
+    """The fused C2R takes the LDS-DMA kernel where the stage allows it (the GPU parity test
+    of that kernel runs these shapes), and it fits the LDS with or without the LDS twiddles."""
+    D.set_config(enable_fused=True, twl=twl)
+    p = D.PlanR2C(list(dims), prec, D.Effort.ESTIMATE, D.Executor.HIP)
+    p.report()
+    out = capfd.readouterr().out
+    line = [ln for ln in out.splitlines() if "+c2r" in ln][0]
+    assert ("fft_c2r_dma" in line) == dma, line
