@@ -395,24 +395,48 @@ struct fft_tile {
 // passes read the twiddles from global memory.  TWL = true: persistent blocks (as many as are
 // resident) copy the twiddle table into LDS once; used for long lines, whose table would
 // otherwise share the L1 cache with the tile data.
+//
+// Column-in launches map tiles to blocks XCD by XCD.  Workgroups are dealt to the 8 XCDs of
+// the chip in turn (block b on XCD b % 8), and each XCD has its own L2.  A column tile reads
+// one LPB-line piece of every row; when rows are not a multiple of 128 bytes long (513 lines
+// of an R2C pencil) every piece straddles two 128-byte lines, whose other halves belong to
+// the neighbouring tiles.  Dealt in tile order, neighbours sit on different XCDs and each
+// fetches both lines (about twice the bytes).  So within each round of G = gridDim.x slots,
+// XCD x takes the G / 8 consecutive tiles [x G / 8, (x + 1) G / 8), and neighbours share an
+// L2.  The slots run over whole rounds so that the map is a permutation of the tiles.
+namespace dtfft {
+__device__ __forceinline__ i64 xcd_tile(i64 slot, i64 G) {
+    if (G % 8 != 0) return slot;
+    const i64 r = slot / G, b = slot % G;
+    return r * G + (b % 8) * (G / 8) + b / 8;
+}
+}  // namespace dtfft
+
 template <typename T, class S, int TPL, int LPB, bool IN_COLS, bool OUT_COLS, int PAD, int SEG = 0, bool TWL = false>
 __global__ void __launch_bounds__(TPL * LPB)
 fft_c2c_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
                const typename tw_of<T>::type* __restrict__ tw, dtfft::lines_desc d, int inv) {
     typedef dtfft::fft_tile<T, S, TPL, LPB, IN_COLS, OUT_COLS, PAD, SEG> F;
+    typedef typename tw_of<T>::type tw_t;
     __shared__ cx<T> lds[F::LDS_ELEMS];
+    const tw_t* twp = tw;
     if constexpr (TWL) {
-        typedef typename tw_of<T>::type tw_t;
         __shared__ tw_t twl[F::TW_SIZE > 0 ? F::TW_SIZE : 1];
         for (int i = threadIdx.x; i < F::TW_SIZE; i += blockDim.x) twl[i] = tw[i];
         __syncthreads();
-        for (i64 t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
-            F::run(in, out, twl, lds, d, t, inv != 0);
+        twp = twl;
+    }
+    const i64 G = gridDim.x;
+    if constexpr (IN_COLS) {
+        const i64 slots = (d.ntiles + G - 1) / G * G;
+        for (i64 s = blockIdx.x; s < slots; s += G) {
+            const i64 t = dtfft::xcd_tile(s, G);
+            if (t < d.ntiles) F::run(in, out, twp, lds, d, t, inv != 0);
             __syncthreads();
         }
     } else {
-        for (i64 t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
-            F::run(in, out, tw, lds, d, t, inv != 0);
+        for (i64 t = blockIdx.x; t < d.ntiles; t += G) {
+            F::run(in, out, twp, lds, d, t, inv != 0);
             __syncthreads();
         }
     }
