@@ -261,8 +261,9 @@ TRAFFIC_FILE = "profiles/fft_512c64_pmc_traffic.md"
 
 
 def _profiled_traffic():
-    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (copy-kernel calibrated); None when the summary is absent."""
+    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3 summary of the
+    L2-to-memory request counters (each request weighed by its size); None when the summary is
+    absent."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TRAFFIC_FILE)
     try:
         for ln in open(path):
