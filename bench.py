@@ -418,14 +418,22 @@ def config4(D, args, timer):
     sizes = [48, 60, 64, 84, 96, 105, 128, 160, 192, 210, 256, 320, 384]
     rng = np.random.default_rng(20250104)
     grids = [list(map(int, rng.choice(sizes, 3))) for _ in range(64)]
+    # the autotuned cycle (tile shapes and radix schedules timed per length group, at least
+    # MEASURE) is the value; the default shapes (ESTIMATE) are reported beside it
     t0 = time.time()
-    cy = D.Cycle(grids, D.Precision.SINGLE, D.Effort(args.effort))
-    log(f"[bench] cycle with 64 grids created in {time.time() - t0:.1f} s")
+    cy0 = D.Cycle(grids, D.Precision.SINGLE, D.Effort.ESTIMATE)
+    t1 = time.time()
+    tuned_effort = D.Effort(max(int(args.effort), int(D.Effort.MEASURE)))
+    cy = D.Cycle(grids, D.Precision.SINGLE, tuned_effort)
+    t2 = time.time()
+    log(f"[bench] cycle with 64 grids created in {t1 - t0:.1f} s (default), {t2 - t1:.1f} s ({tuned_effort.name})")
     offs, total = cy.layout()
     a, b, aux, c = (D.DeviceArray(total * 8) for _ in range(4))
     a.upload(rng.uniform(-1, 1, 2 * total).astype(np.float32))
     algo = sum(12 * g[0] * g[1] * g[2] * 8 for g in grids)
+    ms_d = _time(D, timer, lambda: cy0.execute_graph(a, b, aux, c), args.steps, args.warmup)
     ms_b = _time(D, timer, lambda: cy.execute_graph(a, b, aux, c), args.steps, args.warmup)
+    log(f"[bench] batched cycle {ms_b:.3f} ms tuned, {ms_d:.3f} ms default shapes")
     fwd_err = None
     if not args.no_cpu:
         fwd_err = check_grids(grids, offs, a.download(total, np.complex64), b.download(total, np.complex64))
@@ -440,8 +448,12 @@ def config4(D, args, timer):
             p.execute(b.ptr + ob, c.ptr + ob, D.Execute.BACKWARD, aux.ptr + ob)
     ms_p = _time(D, timer, plain, max(3, args.steps // 4), 2)
     _line("batched uneven fwd+bwd GB/s, 64 C2C complex64 grids", algo / (ms_b * 1e-3) / 1e9, "GB/s", ms_b, args,
-          "64 grids, axes drawn (seed 20250104) from {48..384}, persistent launch per stage + hipGraph",
+          "64 grids, axes drawn (seed 20250104) from {48..384}, persistent launch per stage + hipGraph, "
+          f"tile shapes and radix schedules from the autotuner ({tuned_effort.name})",
           dtype="c64", algorithmic_bytes=algo,
+          default_shapes={"value": round(algo / (ms_d * 1e-3) / 1e9, 2), "ms_per_step": round(ms_d, 4),
+                          "form": "the same cycle with the default tile shapes and schedules (ESTIMATE)"},
+          tuning_seconds=round(t2 - t1, 1),
           plain={"value": round(algo / (ms_p * 1e-3) / 1e9, 2), "ms_per_step": round(ms_p, 4),
                  "form": "64 sequential plain plans (FFT + permute launches)"},
           batched_over_plain=round(ms_p / ms_b, 3), forward_rel_l2_worst_grid=fwd_err,

@@ -15,8 +15,10 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <atomic>
 #include <memory>
 #include <numeric>
+#include <thread>
 
 #include "internal.h"
 
@@ -111,7 +113,160 @@ static std::string tile_type(const fft_config& c) {
     return s;
 }
 
-static int32_t build_stage(cycle_t* cy, int dir, int s) {
+static int32_t launch_group(const length_group& grp, int inv, const void* in, void* out, hipStream_t st);
+
+static std::string group_expr(const fft_config& cf, bool dbl) {
+    return std::string("batched_fft_kernel<") + (dbl ? "double" : "float") + ", " + std::to_string(cf.tpl * cf.lpb) +
+           ", " + tile_type(cf) + ">";
+}
+
+// One launch of a stage for the grids `members`, all of one axis length, with tile shape cf.
+static int32_t make_group(const cycle_t* cy, const std::vector<stage_spec>& specs, const std::vector<int>& members,
+                          const fft_config& cf, std::unique_ptr<length_group>& out) {
+    auto grp = std::make_unique<length_group>();
+    grp->cfg = cf;
+    grp->block = (unsigned)(cf.tpl * cf.lpb);
+    TRY(rtc_get("fft", group_expr(cf, cy->dbl), &grp->k));
+    TRY(get_twiddles(cf.n, cf.radices, cy->dbl, &grp->tw));
+    std::vector<grid_stage_h> gs;
+    std::vector<int> flat;
+    for (int g : members) {
+        const stage_spec& q = specs[g];
+        grid_stage_h h{};
+        h.d = make_lines(q.nl, q.nb, q.in_ls, q.in_bs, q.in_ns, q.out_ls, q.out_bs, q.out_ns, cf.lpb);
+        h.in_off = cy->off[g];
+        h.out_off = cy->off[g];
+        h.tw = grp->tw->ptr;
+        h.cls = 0;
+        const int idx = (int)gs.size();
+        for (int64_t t = 0; t < h.d.ntiles; t++) { flat.push_back(idx); flat.push_back((int)t); }
+        grp->work += q.n * q.nl * q.nb;
+        gs.push_back(h);
+    }
+    grp->nitems = (int)(flat.size() / 2);
+    if (!compile_only()) {
+        HIP_TRY(hipMalloc(&grp->d_gs, sizeof(grid_stage_h) * gs.size()));
+        HIP_TRY(hipMemcpy(grp->d_gs, gs.data(), sizeof(grid_stage_h) * gs.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMalloc(&grp->d_items, sizeof(int) * flat.size()));
+        HIP_TRY(hipMemcpy(grp->d_items, flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice));
+        int per_cu = 1;
+        if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, grp->k->fn, (int)grp->block, 0) !=
+                hipSuccess || per_cu < 1)
+            per_cu = 1;
+        const int64_t resident = (int64_t)per_cu * std::max(1, device().num_cus);
+        grp->grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(grp->nitems, resident));
+    }
+    out = std::move(grp);
+    return DTFFT_HIP_SUCCESS;
+}
+
+// Candidate tile shapes of a group for the autotuner, the default first: radix schedules of
+// the length (radix_schedules), threads per line of one or two butterflies of the largest
+// radix per pass, and the default's lines per block, halved and doubled.
+static std::vector<fft_config> group_candidates(int n, bool dbl, bool in_cols, bool out_cols, int effort) {
+    std::vector<fft_config> out;
+    const fft_config base = default_fft_config(n, dbl, in_cols, out_cols, in_cols && out_cols);
+    out.push_back(base);
+    const auto scheds = radix_schedules(n, effort >= DTFFT_HIP_EXHAUSTIVE ? 6 : 3);
+    for (const auto& r : scheds) {
+        const int rmax = *std::max_element(r.begin(), r.end());
+        for (int t : {n / rmax, n / rmax / 2})
+            for (int l : {base.lpb, base.lpb * 2, base.lpb / 2}) {
+                if (t < 1 || l < 1 || t * l > 1024 || t * l < 64) continue;
+                fft_config c = base;
+                c.radices = r;
+                c.tpl = t;
+                c.lpb = l;
+                if (fft_lds_bytes(c) > 64 * 1024) continue;  // leave room for several blocks per CU
+                bool dup = false;
+                for (auto& o : out) dup |= (o.radices == c.radices && o.tpl == c.tpl && o.lpb == c.lpb);
+                if (!dup) out.push_back(c);
+            }
+    }
+    if (effort < DTFFT_HIP_EXHAUSTIVE && out.size() > 8) out.resize(8);
+    return out;
+}
+
+static uint64_t fnv1a_ints(const std::vector<int64_t>& v) {
+    uint64_t h = 1469598103934665603ull;
+    for (int64_t x : v)
+        for (int b = 0; b < 8; b++) { h ^= (uint64_t)((x >> (8 * b)) & 0xff); h *= 1099511628211ull; }
+    return h;
+}
+
+// At effort >= MEASURE: time the candidate shapes of one group on scratch buffers and keep the
+// fastest (the project's replacement of dtfft_nvrtc_block_optimizer for the batched cycle).
+// The choice is stored as a tuning record keyed by the group's length, layout and lines.
+static int32_t tune_group(cycle_t* cy, const std::vector<stage_spec>& specs, const std::vector<int>& members,
+                          bool in_cols, bool out_cols, void* scratch_in, void* scratch_out, fft_config& best) {
+    const int n = (int)specs[members[0]].n;
+    std::vector<int64_t> sig;
+    for (int g : members) {
+        const stage_spec& q = specs[g];
+        sig.insert(sig.end(), {q.nl, q.nb, q.in_ls, q.in_bs, q.in_ns, q.out_ls, q.out_bs, q.out_ns});
+    }
+    char kb[160];
+    std::snprintf(kb, sizeof kb, "cyc|%d|%c|%d%d|%zu|%016llx", n, cy->dbl ? 'd' : 'f', (int)in_cols, (int)out_cols,
+                  members.size(), (unsigned long long)fnv1a_ints(sig));
+    const std::string key(kb);
+    best = default_fft_config(n, cy->dbl, in_cols, out_cols, in_cols && out_cols);
+    std::string rec;
+    if (tune_lookup(key, rec)) {
+        int t, l, p;
+        char rads[128] = {0};
+        std::vector<int> r;
+        if (std::sscanf(rec.c_str(), "%d %d %d %127s", &t, &l, &p, rads) == 4 && parse_radices(rads, n, r)) {
+            fft_config c = best;
+            c.tpl = t; c.lpb = l; c.pad = p; c.radices = r;
+            if (t * l >= 1 && t * l <= 1024 && fft_lds_bytes(c) <= 160 * 1024) best = c;
+        }
+        return DTFFT_HIP_SUCCESS;
+    }
+    if (cy->effort < DTFFT_HIP_MEASURE || compile_only() || !scratch_in) return DTFFT_HIP_SUCCESS;
+    auto cands = group_candidates(n, cy->dbl, in_cols, out_cols, cy->effort);
+    // compile the candidates side by side (hipRTC programs are independent)
+    {
+        std::vector<std::thread> pool;
+        std::atomic<size_t> next{0};
+        const unsigned nt = std::max(1u, std::min<unsigned>(8, std::thread::hardware_concurrency()));
+        for (unsigned i = 0; i < nt; i++)
+            pool.emplace_back([&]() {
+                for (size_t j; (j = next++) < cands.size();) {
+                    const rtc_kernel* k = nullptr;
+                    rtc_get("fft", group_expr(cands[j], cy->dbl), &k);
+                }
+            });
+        for (auto& th : pool) th.join();
+    }
+    double best_ms = 1e30;
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    for (const auto& c : cands) {
+        std::unique_ptr<length_group> grp;
+        if (make_group(cy, specs, members, c, grp) != DTFFT_HIP_SUCCESS) continue;
+        bool ok = true;
+        for (int i = 0; i < 2 && ok; i++) ok = launch_group(*grp, 0, scratch_in, scratch_out, cy->stream) == 0;
+        if (!ok) continue;
+        hipEventRecord(a, cy->stream);
+        for (int i = 0; i < 5; i++) launch_group(*grp, 0, scratch_in, scratch_out, cy->stream);
+        hipEventRecord(b, cy->stream);
+        hipEventSynchronize(b);
+        float ms = 0;
+        hipEventElapsedTime(&ms, a, b);
+        if (ms < best_ms) { best_ms = ms; best = c; }
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    std::string r;
+    for (size_t i = 0; i < best.radices.size(); i++) r += (i ? "," : "") + std::to_string(best.radices[i]);
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "%d %d %d %s", best.tpl, best.lpb, best.pad, r.c_str());
+    tune_store(key, buf);
+    return DTFFT_HIP_SUCCESS;
+}
+
+static int32_t build_stage(cycle_t* cy, int dir, int s, void* scratch_in, void* scratch_out) {
     cycle_stage& cs = cy->st[dir][s];
     std::vector<stage_spec> specs(cy->ng);
     std::vector<int> group_of(cy->ng, -1);
@@ -131,47 +286,15 @@ static int32_t build_stage(cycle_t* cy, int dir, int s) {
     // every grid of a stage has the same layout kind (rows or columns on each side)
     std::vector<std::unique_ptr<length_group>> groups;
     for (size_t c = 0; c < lengths.size(); c++) {
-        auto grp = std::make_unique<length_group>();
-        int g0 = (int)(std::find(group_of.begin(), group_of.end(), (int)c) - group_of.begin());
-        const stage_spec& sp = specs[g0];
+        std::vector<int> members;
+        for (int g = 0; g < cy->ng; g++) if (group_of[g] == (int)c) members.push_back(g);
+        const stage_spec& sp = specs[members[0]];
         const bool in_cols = sp.in_ns != 1 && sp.in_ls == 1;
         const bool out_cols = sp.out_ns != 1 && sp.out_ls == 1;
-        grp->cfg = default_fft_config((int)sp.n, cy->dbl, in_cols, out_cols, in_cols && out_cols);
-        const fft_config& cf = grp->cfg;
-        grp->block = (unsigned)(cf.tpl * cf.lpb);
-        const std::string expr = std::string("batched_fft_kernel<") + (cy->dbl ? "double" : "float") + ", " +
-                                 std::to_string(grp->block) + ", " + tile_type(cf) + ">";
-        TRY(rtc_get("fft", expr, &grp->k));
-        TRY(get_twiddles(cf.n, cf.radices, cy->dbl, &grp->tw));
-        std::vector<grid_stage_h> gs;
-        std::vector<int> flat;
-        for (int g = 0; g < cy->ng; g++) {
-            if (group_of[g] != (int)c) continue;
-            const stage_spec& q = specs[g];
-            grid_stage_h h{};
-            h.d = make_lines(q.nl, q.nb, q.in_ls, q.in_bs, q.in_ns, q.out_ls, q.out_bs, q.out_ns, cf.lpb);
-            h.in_off = cy->off[g];
-            h.out_off = cy->off[g];
-            h.tw = grp->tw->ptr;
-            h.cls = 0;
-            const int idx = (int)gs.size();
-            for (int64_t t = 0; t < h.d.ntiles; t++) { flat.push_back(idx); flat.push_back((int)t); }
-            grp->work += q.n * q.nl * q.nb;
-            gs.push_back(h);
-        }
-        grp->nitems = (int)(flat.size() / 2);
-        if (!compile_only()) {
-            HIP_TRY(hipMalloc(&grp->d_gs, sizeof(grid_stage_h) * gs.size()));
-            HIP_TRY(hipMemcpy(grp->d_gs, gs.data(), sizeof(grid_stage_h) * gs.size(), hipMemcpyHostToDevice));
-            HIP_TRY(hipMalloc(&grp->d_items, sizeof(int) * flat.size()));
-            HIP_TRY(hipMemcpy(grp->d_items, flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice));
-            int per_cu = 1;
-            if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, grp->k->fn, (int)grp->block, 0) !=
-                    hipSuccess || per_cu < 1)
-                per_cu = 1;
-            const int64_t resident = (int64_t)per_cu * std::max(1, device().num_cus);
-            grp->grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(grp->nitems, resident));
-        }
+        fft_config cf;
+        TRY(tune_group(cy, specs, members, in_cols, out_cols, scratch_in, scratch_out, cf));
+        std::unique_ptr<length_group> grp;
+        TRY(make_group(cy, specs, members, cf, grp));
         groups.push_back(std::move(grp));
     }
     std::stable_sort(groups.begin(), groups.end(),
@@ -259,8 +382,19 @@ int32_t dtfft_hip_cycle_create(int32_t n_grids, const int32_t* dims, int32_t pre
     cy->total = off;
     if (!compile_only())
         for (auto& x : cy->side) HIP_TRY(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
-    for (int dir = 0; dir < 2; dir++)
-        for (int s = 0; s < 3; s++) TRY(build_stage(cy.get(), dir, s));
+    // scratch buffers for timing candidate tile shapes (effort >= MEASURE), freed after creation
+    void *si = nullptr, *so = nullptr;
+    if (effort >= DTFFT_HIP_MEASURE && !compile_only()) {
+        const size_t bytes = (size_t)cy->total * (cy->dbl ? 16 : 8);
+        HIP_TRY(hipMalloc(&si, bytes));
+        if (hipMalloc(&so, bytes) != hipSuccess) { hipFree(si); return DTFFT_HIP_ERROR_ALLOC_FAILED; }
+        if (hipMemsetAsync(si, 0, bytes, cy->stream) != hipSuccess) { hipFree(si); hipFree(so); return DTFFT_HIP_ERROR_HIP_RUNTIME; }
+    }
+    int32_t rc = DTFFT_HIP_SUCCESS;
+    for (int dir = 0; dir < 2 && rc == DTFFT_HIP_SUCCESS; dir++)
+        for (int s = 0; s < 3 && rc == DTFFT_HIP_SUCCESS; s++) rc = build_stage(cy.get(), dir, s, si, so);
+    if (si) { hipStreamSynchronize(cy->stream); hipFree(si); hipFree(so); }
+    TRY(rc);
     *cycle = cy.release();
     return DTFFT_HIP_SUCCESS;
 }

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <functional>
 
 #include "internal.h"
 
@@ -216,24 +217,93 @@ fft_config default_fft_config(int n, bool dbl, bool in_cols, bool out_cols, bool
     return c;
 }
 
+// Radix schedules of n for the autotuner, the default (factorize) first: the power-of-two
+// part split into the fewest passes of radix <= 16 or one more, in descending and ascending
+// order, with the odd primes after (default) or before it.  At most `max_count`.
+std::vector<std::vector<int>> radix_schedules(int n, int max_count) {
+    std::vector<std::vector<int>> out;
+    std::vector<int> def;
+    if (!factorize(n, def)) return out;
+    out.push_back(def);
+    int a = 0, m = n;
+    while (m % 2 == 0) { m /= 2; a++; }
+    std::vector<int> odd;
+    for (int r : def)
+        if (r % 2 != 0 && r != 1) odd.push_back(r);
+    std::vector<std::vector<int>> pow2;  // exponent lists, each in [1, 4]
+    if (a == 0) pow2.push_back({});
+    const int kmin = (a + 3) / 4;
+    for (int k = kmin; a > 0 && k <= std::min(a, kmin + 1); k++) {
+        // multisets of k exponents in [1, 4] summing to a, descending
+        std::vector<int> e(k, 1);
+        std::function<void(int, int, int)> rec = [&](int i, int left, int hi) {
+            if (i == k) { if (left == 0) pow2.push_back(e); return; }
+            for (int x = std::min(hi, 4); x >= 1; x--) {
+                if (x > left - (k - i - 1)) continue;
+                e[i] = x;
+                rec(i + 1, left - x, x);
+            }
+        };
+        rec(0, a, 4);
+    }
+    auto push = [&](const std::vector<int>& r) {
+        if ((int)out.size() >= max_count || r.empty() || r.size() > 10) return;
+        for (auto& o : out) if (o == r) return;
+        out.push_back(r);
+    };
+    for (auto& ex : pow2) {
+        std::vector<int> p2;
+        for (int x : ex) p2.push_back(1 << x);
+        std::vector<int> asc(p2.rbegin(), p2.rend());
+        for (const auto* seq : {&p2, &asc}) {
+            std::vector<int> r = *seq;
+            r.insert(r.end(), odd.begin(), odd.end());
+            push(r);
+            std::vector<int> f = odd;
+            f.insert(f.end(), seq->begin(), seq->end());
+            push(f);
+        }
+    }
+    return out;
+}
+
 std::vector<fft_config> fft_candidates(int n, bool dbl, bool in_cols, bool out_cols, bool inplace) {
     std::vector<fft_config> out;
     fft_config base = default_fft_config(n, dbl, in_cols, out_cols, inplace);
     out.push_back(base);
-    std::vector<int> tpls, lpbs;
-    for (int t = base.tpl; t >= 1 && (int)tpls.size() < 3; t /= 2) tpls.push_back(t);
+    std::vector<int> lpbs;
     if (in_cols || out_cols) lpbs = {4, 8, 16, 32};
     else lpbs = {1, 2, 4, 8, 16, 32, 64};
+    auto add = [&](const fft_config& c) {
+        if (!valid(c) || c.tpl * c.lpb < 64) return;
+        for (auto& o : out)
+            if (o.tpl == c.tpl && o.lpb == c.lpb && o.pad == c.pad && o.radices == c.radices) return;
+        out.push_back(c);
+    };
+    // other radix schedules first (MEASURE times the first 12 candidates), each with the
+    // threads per line of its own largest radix (one and two butterflies per thread and pass)
+    // and the default's lines per block and padding
+    const auto scheds = radix_schedules(n, 4);
+    for (size_t i = 1; i < scheds.size(); i++) {
+        const int rmax = *std::max_element(scheds[i].begin(), scheds[i].end());
+        for (int t : {n / rmax, n / rmax / 2}) {
+            if (t < 1) continue;
+            fft_config c = base;
+            c.radices = scheds[i];
+            c.tpl = t;
+            while (c.tpl * c.lpb > 1024 && c.lpb > 1) c.lpb /= 2;
+            add(c);
+        }
+    }
+    // launch shapes of the default schedule
+    std::vector<int> tpls;
+    for (int t = base.tpl; t >= 1 && (int)tpls.size() < 3; t /= 2) tpls.push_back(t);
     for (int t : tpls)
         for (int l : lpbs)
             for (int p : {0, 1, 2, 3}) {
                 fft_config c = base;
                 c.tpl = t; c.lpb = l; c.pad = p;
-                if (!valid(c)) continue;
-                if (c.tpl * c.lpb < 64) continue;
-                bool dup = false;
-                for (auto& o : out) dup |= (o.tpl == c.tpl && o.lpb == c.lpb && o.pad == c.pad);
-                if (!dup) out.push_back(c);
+                add(c);
             }
     return out;
 }
@@ -485,6 +555,25 @@ int32_t fft_stage::run(const void* in, void* out, int sign, hipStream_t s) const
     return launch(k, grid, block, s, args);
 }
 
+// "r0,r1,..." -> radices whose product is n, each a radix the kernels have
+bool parse_radices(const char* text, int n, std::vector<int>& out) {
+    out.clear();
+    long prod = 1;
+    const char* p = text;
+    while (*p) {
+        char* end = nullptr;
+        const long r = std::strtol(p, &end, 10);
+        if (end == p) return false;
+        if (r != 2 && r != 3 && r != 4 && r != 5 && r != 7 && r != 8 && r != 11 && r != 13 && r != 16) return false;
+        out.push_back((int)r);
+        prod *= r;
+        p = end;
+        if (*p == ',') p++;
+        else if (*p) return false;
+    }
+    return !out.empty() && out.size() <= 10 && prod == n;
+}
+
 static double time_stage(const fft_stage& st, void* in, void* out, hipStream_t s, int warm, int iters) {
     for (int i = 0; i < warm; i++) if (st.run(in, out, -1, s)) return 1e30;
     hipEvent_t a, b;
@@ -534,11 +623,16 @@ int32_t plan_fft_stage(int n, bool dbl, int64_t nl, int64_t nb, int64_t in_ls, i
     int rec_twl = -1;  // twl of a tuning record (-1: the record predates the field)
     if (n > 1 && tune_lookup(key, rec)) {
         tuned = true;
-        int t, l, p, dm = 0, tw = -1;
-        if (std::sscanf(rec.c_str(), "%d %d %d %d %d", &t, &l, &p, &dm, &tw) >= 3) {
+        int t, l, p, dm = 0, tw = -1, used = 0;
+        char rads[128] = {0};
+        const int got = std::sscanf(rec.c_str(), "%d %d %d %d %d %n%127s", &t, &l, &p, &dm, &tw, &used, rads);
+        if (got >= 3) {
             fft_config c = cfg;
             c.tpl = t; c.lpb = l; c.pad = p;
             c.dma = dm;
+            // the radix schedule, "r0,r1,..." (records without one keep the default)
+            std::vector<int> r;
+            if (got >= 6 && parse_radices(rads, n, r)) c.radices = r;
             if (valid(c) && (!c.dma || dma_ok(c, d))) { cfg = c; rec_twl = tw; }
         }
     } else if (n > 1 && effort >= DTFFT_HIP_MEASURE && !compile_only()) {
@@ -578,8 +672,10 @@ int32_t plan_fft_stage(int n, bool dbl, int64_t nl, int64_t nb, int64_t in_ls, i
         hipStreamSynchronize(stream);
         hipFree(bi);
         if (!inplace) hipFree(bo);
-        char buf[64];
-        std::snprintf(buf, sizeof buf, "%d %d %d %d %d", cfg.tpl, cfg.lpb, cfg.pad, cfg.dma, cfg.twl);
+        std::string r;
+        for (size_t i = 0; i < cfg.radices.size(); i++) r += (i ? "," : "") + std::to_string(cfg.radices[i]);
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "%d %d %d %d %d %s", cfg.tpl, cfg.lpb, cfg.pad, cfg.dma, cfg.twl, r.c_str());
         tune_store(key, buf);
         tuned = true;
         rec_twl = cfg.twl;

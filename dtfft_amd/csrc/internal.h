@@ -121,6 +121,10 @@ void apply_twl(fft_config& c, const lines_desc& d, int mode);
 // Chooses tpl / lpb for a length and access mode (heuristic default of the autotuner).
 fft_config default_fft_config(int n, bool dbl, bool in_cols, bool out_cols, bool inplace);
 std::vector<fft_config> fft_candidates(int n, bool dbl, bool in_cols, bool out_cols, bool inplace);
+// Radix schedules of n for the autotuner (the default first), at most max_count.
+std::vector<std::vector<int>> radix_schedules(int n, int max_count);
+// Parses a tuning record's "r0,r1,..." schedule of n.
+bool parse_radices(const char* text, int n, std::vector<int>& out);
 
 // One batched C2C stage: y = FFT_sign(x) over the lines of `d`.
 struct fft_stage {

@@ -103,3 +103,37 @@ def test_cycle_64_uneven_round_trip():
         nx, ny, nz = grids[i]
         z = out[offs[i]:offs[i] + xs[i].size].reshape(ny, nx, nz).astype(np.complex128)
         assert rel(z, OP.execute_generic(xs[i].astype(np.complex128), True)) < 1e-6, grids[i]
+
+
+@pytest.mark.parametrize("effort", [D.Effort.MEASURE, D.Effort.EXHAUSTIVE])
+def test_tuned_cycle_against_oracle(tmp_path, effort):
+    """At effort >= MEASURE the batched cycle times candidate tile shapes and radix schedules
+    for every length group and keeps the fastest (tuning records "cyc|..."); the tuned cycle
+    must match the oracle grid by grid, and a second cycle reads the records back."""
+    import os
+    L = D.lib()
+    default_dir = os.environ.get("DTFFT_HIP_CACHE_DIR") or os.path.join(os.path.dirname(D._lib.LIB_PATH), "kcache")
+    assert L.dtfft_hip_set_cache_dir(str(tmp_path).encode()) == 0
+    try:
+        grids = [[48, 30, 20], [64, 48, 60], [105, 32, 16], [384, 16, 12], [320, 20, 16]]
+        for rnd in range(2):
+            cy = D.Cycle(grids, D.Precision.SINGLE, effort)
+            offs, total = cy.layout()
+            rng = np.random.default_rng(9 + rnd)
+            buf, xs = pack(grids, offs, total, rng, np.complex64)
+            a = D.DeviceArray.from_numpy(buf)
+            b, aux, c = D.DeviceArray(total * 8), D.DeviceArray(total * 8), D.DeviceArray(total * 8)
+            cy.execute(a, b, aux, D.Execute.FORWARD)
+            out = b.download(total, np.complex64)
+            for g, o, x in zip(grids, offs, xs):
+                nx, ny, nz = g
+                z = out[o:o + x.size].reshape(ny, nx, nz).astype(np.complex128)
+                assert rel(z, OP.execute_generic(x.astype(np.complex128), True)) < 1e-6, g
+            cy.execute(b, c, aux, D.Execute.BACKWARD)
+            back = c.download(total, np.complex64)
+            for g, o, x in zip(grids, offs, xs):
+                assert rel(back[o:o + x.size].astype(np.complex128) / x.size, x.ravel()) < 1e-5, g
+        recs = [ln for ln in (tmp_path / "tuning.txt").read_text().splitlines() if ln.startswith("cyc|")]
+        assert len(recs) >= 6
+    finally:
+        L.dtfft_hip_set_cache_dir(default_dir.encode())

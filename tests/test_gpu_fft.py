@@ -138,3 +138,55 @@ def test_tuning_records_select_spread_lds_layout(tmp_path, prec):
         assert rel(b2.download(y.shape, cdt).astype(np.complex128), want) < tol
     finally:
         L.dtfft_hip_set_cache_dir(default_dir.encode())
+
+
+@pytest.mark.parametrize("prec", [D.Precision.SINGLE, D.Precision.DOUBLE])
+@pytest.mark.parametrize("sched,tpl", [("16,8,4", 32), ("4,8,16", 64), ("2,16,16", 16)])
+def test_tuning_records_select_radix_schedule(tmp_path, prec, sched, tpl):
+    """A tuning record's radix schedule (the autotuner's sixth field) is honoured: a 512-point
+    row stage run as 16 x 8 x 4, 4 x 8 x 16 or 2 x 16 x 16 must match the oracle both ways."""
+    L = D.lib()
+    c = "d" if prec == D.Precision.DOUBLE else "f"
+    n, hm = 512, 96
+    key = f"fft|{n}|{c}|000|{hm}|1|{n},0,1|{n},0,1"
+    (tmp_path / "tuning.txt").write_text(f"{key}\t{tpl} 2 0 0 0 {sched}\n")
+    default_dir = os.environ.get("DTFFT_HIP_CACHE_DIR") or os.path.join(os.path.dirname(D._lib.LIB_PATH), "kcache")
+    assert L.dtfft_hip_set_cache_dir(str(tmp_path).encode()) == 0
+    try:
+        rng = np.random.default_rng(11)
+        cdt = np.complex128 if prec == D.Precision.DOUBLE else np.complex64
+        tol = 1e-12 if prec == D.Precision.DOUBLE else 1e-6
+        x = (rng.uniform(-1, 1, (hm, n)) + 1j * rng.uniform(-1, 1, (hm, n))).astype(cdt)
+        ex = D.FFTExecutor(1, D.FFTType.C2C, prec, n, n, hm, [n])
+        a = D.DeviceArray.from_numpy(x)
+        b = D.DeviceArray(x.nbytes)
+        for sign in (D.FFT_FORWARD, D.FFT_BACKWARD):
+            ex.execute(a, b, sign)
+            got = b.download(x.shape, cdt).astype(np.complex128)
+            assert rel(got, O.fft_c2c(x.astype(np.complex128), sign)) < tol
+    finally:
+        L.dtfft_hip_set_cache_dir(default_dir.encode())
+
+
+@pytest.mark.parametrize("n", [384, 320, 48])
+def test_exhaustive_stage_with_schedule_candidates_is_correct(tmp_path, n):
+    """At EXHAUSTIVE the autotuner also times other radix schedules (16 x 8 x 3, 3 x 16 x 8,
+    ...); whichever wins must match the oracle, and its record names a schedule of n."""
+    L = D.lib()
+    default_dir = os.environ.get("DTFFT_HIP_CACHE_DIR") or os.path.join(os.path.dirname(D._lib.LIB_PATH), "kcache")
+    assert L.dtfft_hip_set_cache_dir(str(tmp_path).encode()) == 0
+    try:
+        hm = 256
+        rng = np.random.default_rng(n)
+        x = (rng.uniform(-1, 1, (hm, n)) + 1j * rng.uniform(-1, 1, (hm, n))).astype(np.complex64)
+        ex = D.FFTExecutor(1, D.FFTType.C2C, D.Precision.SINGLE, n, n, hm, [n], effort=D.Effort.EXHAUSTIVE)
+        a = D.DeviceArray.from_numpy(x)
+        b = D.DeviceArray(x.nbytes)
+        ex.execute(a, b, D.FFT_FORWARD)
+        assert rel(b.download(x.shape, x.dtype).astype(np.complex128), O.fft_c2c(x.astype(np.complex128))) < 1e-6
+        rec = [ln for ln in (tmp_path / "tuning.txt").read_text().splitlines() if ln.startswith(f"fft|{n}|")]
+        assert rec
+        rads = [int(r) for r in rec[0].split("\t")[1].split()[5].split(",")]
+        assert int(np.prod(rads)) == n
+    finally:
+        L.dtfft_hip_set_cache_dir(default_dir.encode())
