@@ -464,6 +464,42 @@ class PencilTranspose:
         self.share.run_unpack(lambda i: self.unpack.execute(recv, out, neighbor=i, stream=stream))
 
 
+class PencilReshape:
+    """One rank's bricks <-> pencils reshape on the GPU (src/dtfft_reshape_plan.F90:98-260 with
+    the reshape branches of reshape_handle_generic, restated in pencil.ReshapeShare): PACK /
+    UNPACK kernels with their neighbour data, or none where the layout is pack-free /
+    unpack-free, in the pack -> exchange -> unpack order.  The exchange is the caller's."""
+
+    def __init__(self, reshape_type: int, dims, comm_size: int, me: int, base_storage: int, mode: str = "mpi",
+                 split=None, effort: Effort = Effort.ESTIMATE):
+        from . import pencil
+        self.share = sh = pencil.ReshapeShare(reshape_type, dims, comm_size, me, mode, split)
+        self.es = int(base_storage)
+        self.pack = self.unpack = None
+        if sh.pack_kernel != pencil.KERNEL_DUMMY:
+            self.pack = Kernel(sh.pack_kernel, sh.in_counts, self.es, sh.send_rows, effort)
+        if sh.unpack_kernel != pencil.KERNEL_DUMMY:
+            self.unpack = Kernel(sh.unpack_kernel, sh.out_counts, self.es, sh.recv_rows, effort)
+
+    @property
+    def send_bytes(self) -> int:
+        return sum(self.share.send_counts) * self.es
+
+    @property
+    def recv_bytes(self) -> int:
+        return sum(self.share.recv_counts) * self.es
+
+    def pack_into(self, inp, send, stream: int | None = None) -> None:
+        """With a pack-free layout nothing runs: the input is the send buffer."""
+        if self.pack is not None:
+            self.share.run_pack(lambda i: self.pack.execute(inp, send, neighbor=i, stream=stream))
+
+    def unpack_from(self, recv, out, stream: int | None = None) -> None:
+        """With an unpack-free layout nothing runs: the receive buffer is the output."""
+        if self.unpack is not None:
+            self.share.run_unpack(lambda i: self.unpack.execute(recv, out, neighbor=i, stream=stream))
+
+
 class Cycle:
     """Batched fwd+bwd cycle over many 3D C2C grids (dims per grid in Fortran order)."""
 

@@ -274,3 +274,196 @@ class TransposeShare:
                 unpack(i)
         else:
             unpack(0)
+
+
+# ---------------------------------------------------------------------------------------
+# Bricks <-> pencils reshapes (src/dtfft_reshape_plan.F90:98-260, the reshape branches of
+# src/dtfft_reshape_handle_generic.F90:261-289, 343-403, 535-565, 584-596)
+# ---------------------------------------------------------------------------------------
+# dtfft_reshape_t values (include/dtfft_config.h.in:45-48)
+X_BRICKS_TO_PENCILS, X_PENCILS_TO_BRICKS, Z_PENCILS_TO_BRICKS, Z_BRICKS_TO_PENCILS = 11, 12, 13, 14
+BRICKS_TO_PENCILS = (X_BRICKS_TO_PENCILS, Z_BRICKS_TO_PENCILS)
+KERNEL_DUMMY = -1
+RESHAPE_MODES = ("mpi", "pipelined", "fused")
+
+
+def sub_bricks(reshape_type: int, dims, comm_size: int, split=None):
+    """Bricks and pencils of every rank of the sub-communicator a reshape runs in.
+
+    ``dims`` are the group's extents in the pencil's own axis order (X-pencil [X, Y, Z] for the
+    X reshapes, Z-pencil [Z, X, Y] for the Z reshapes; 2D [X, Y]).  In the bricks the first
+    axis is split over the ``comm_size`` ranks and the others are whole; in the pencils the
+    first axis is whole and the others are split ``split = (a, b)`` ways (a * b = comm_size;
+    2D: a = comm_size), rank i holding part (i % a, i // a): the order of the group's ranks by
+    pencil start that the reshape plan's MPI_Comm_split gives them
+    (src/dtfft_reshape_plan.F90:168-174).  Returns the list over ranks of (in_counts,
+    in_starts, out_counts, out_starts) in that axis order."""
+    nd = len(dims)
+    P = comm_size
+    if split is None:
+        split = (P, 1) if nd == 2 else (1, P)
+    a, b = split
+    if a * b != P or (nd == 2 and b != 1):
+        raise ValueError("split must multiply to the group size (2D: (P, 1))")
+    res = []
+    for r in range(P):
+        xs, xn = local_size(dims[0], P, r)
+        ys, yn = local_size(dims[1], a, r % a)
+        if nd == 2:
+            brick = ((xn, dims[1]), (xs, 0))
+            pen = ((dims[0], yn), (0, ys))
+        else:
+            zs, zn = local_size(dims[2], b, r // a)
+            brick = ((xn, dims[1], dims[2]), (xs, 0, 0))
+            pen = ((dims[0], yn, zn), (0, ys, zs))
+        send, recv = (brick, pen) if reshape_type in BRICKS_TO_PENCILS else (pen, brick)
+        res.append((send[0], send[1], recv[0], recv[1]))
+    return res
+
+
+def reshape_neighbor_data(reshape_type: int, ndims: int, pencils, me: int):
+    """neighbor_data of a reshape's pack and unpack kernels on one rank, the reshape branches
+    of reshape_handle_generic%create: send side src/dtfft_reshape_handle_generic.F90:343-403,
+    receive side :535-565 and :584-596, the strategy of :265-289 (1: every pencil holds the
+    bricks' whole second axis, 2: the whole third axis, 3: neither) and the pack-free /
+    unpack-free tests of :261-264 and :479-484.
+
+    Returns (send_rows, recv_rows, send_counts, recv_counts, strat, pack_free, unpack_free)."""
+    P = len(pencils)
+    in_sizes = [p[0] for p in pencils]
+    in_starts = [p[1] for p in pencils]
+    out_sizes = [p[2] for p in pencils]
+    out_starts = [p[3] for p in pencils]
+    b2p = reshape_type in BRICKS_TO_PENCILS
+    # pack-free (bricks -> pencils) / unpack-free (pencils -> bricks): every peer's pencil part
+    # holds the whole second axis of my brick (the test of every rank agrees, :259-264)
+    second_whole = all(in_sizes[me][1] == out_sizes[i][1] for i in range(P)) or ndims == 2
+    pack_free = b2p and second_whole
+    unpack_free = (not b2p) and second_whole
+    if ndims == 2:
+        strat = 1
+    else:
+        if b2p:
+            zslab = all(in_sizes[me][1] == out_sizes[i][1] for i in range(P))
+            yslab = all(in_sizes[me][2] == out_sizes[i][2] for i in range(P))
+        else:
+            zslab = all(in_sizes[i][1] == out_sizes[me][1] for i in range(P))
+            yslab = all(in_sizes[i][2] == out_sizes[me][2] for i in range(P))
+        strat = 1 if zslab else (2 if yslab else 3)
+
+    def send_ln_ls(src: int, i: int):
+        if ndims == 2:
+            if b2p:
+                return (in_sizes[src][0], out_sizes[i][1]), (in_starts[src][0], out_starts[i][1])
+            return (out_sizes[i][0], in_sizes[src][1]), (out_starts[i][0], in_starts[src][1])
+        if b2p:
+            return ((in_sizes[src][0], out_sizes[i][1], out_sizes[i][2]),
+                    (in_starts[src][0], out_starts[i][1], out_starts[i][2]))
+        return ((out_sizes[i][0], in_sizes[src][1], in_sizes[src][2]),
+                (out_starts[i][0], in_starts[src][1], in_starts[src][2]))
+
+    def prod(v):
+        n = 1
+        for x in v:
+            n *= int(x)
+        return n
+
+    send, scount = [], []
+    sdispl = ssdispl = 0
+    sc, ss = in_sizes[me], in_starts[me]
+    for i in range(P):
+        ln, ls = send_ln_ls(me, i)
+        if b2p:
+            if strat == 1:
+                din = ssdispl
+                ssdispl += prod(ln)
+            elif strat == 2:
+                din = ssdispl
+                ssdispl += ln[0] * ln[1]
+            else:
+                din = (out_starts[i][2] - ss[2]) * sc[0] * sc[1] + (out_starts[i][1] - ss[1]) * sc[0]
+        else:
+            din = ls[0]
+        send.append((ln[0], ln[1], ln[2] if ndims == 3 else 1, din, sdispl))
+        n = prod(ln)
+        scount.append(n)
+        sdispl += n
+
+    recv, rcount = [], []
+    rdispl = rrdispl = 0
+    rc, rs = out_sizes[me], out_starts[me]
+    for i in range(P):
+        recvsize = prod(send_ln_ls(i, me)[0])
+        ln = (0, 0, 0)
+        ls = (0, 0, 0)
+        if recvsize > 0:
+            if ndims == 2:
+                if b2p:
+                    ln, ls = (in_sizes[i][0], out_sizes[me][1], 1), (in_starts[i][0], out_starts[me][1], 0)
+                else:
+                    ln, ls = (out_sizes[me][0], in_sizes[i][1], 1), (out_starts[me][0], in_starts[i][1], 0)
+            elif b2p:
+                ln = (in_sizes[i][0], out_sizes[me][1], out_sizes[me][2])
+                ls = (in_starts[i][0], out_starts[me][1], out_starts[me][2])
+            else:
+                ln = (out_sizes[me][0], in_sizes[i][1], in_sizes[i][2])
+                ls = (out_starts[me][0], in_starts[i][1], in_starts[i][2])
+        if b2p:
+            dout = ls[0]
+        elif strat in (1, 2):
+            dout = rrdispl
+            rrdispl += ln[0] * ln[1]
+        else:
+            dout = abs(rs[2] - in_starts[i][2]) * rc[0] * rc[1] + abs(rs[1] - in_starts[i][1]) * rc[0]
+        recv.append((ln[0], ln[1], ln[2] if ndims == 3 else 1, rdispl, dout))
+        rcount.append(recvsize)
+        rdispl += recvsize
+    return send, recv, scount, rcount, strat, pack_free, unpack_free
+
+
+def reshape_kernel_types(mode: str, pack_free: bool, unpack_free: bool):
+    """(pack kernel, unpack kernel) of a reshape (src/dtfft_reshape_handle_generic.F90:415-419,
+    586-595): PACK / UNPACK, their per-peer forms when fused / pipelined, DUMMY where the
+    layout makes them unnecessary."""
+    if mode not in RESHAPE_MODES:
+        raise ValueError(f"mode must be one of {RESHAPE_MODES}")
+    pack = KERNEL_DUMMY if pack_free else (KERNEL_PACK_PIPELINED if mode == "fused" else KERNEL_PACK)
+    unpack = KERNEL_DUMMY if unpack_free else (KERNEL_UNPACK_PIPELINED if mode in ("pipelined", "fused")
+                                               else KERNEL_UNPACK)
+    return pack, unpack
+
+
+class ReshapeShare:
+    """One rank's bricks <-> pencils reshape: geometry, neighbour data, kernel types, and the
+    pack -> exchange -> unpack order of reshape_handle_generic%execute
+    (src/dtfft_reshape_handle_generic.F90:695-759).  With a DUMMY pack (pack-free) the input
+    is the send buffer; with a DUMMY unpack (unpack-free) the receive buffer is the output."""
+
+    def __init__(self, reshape_type: int, dims, comm_size: int, me: int, mode: str = "mpi", split=None):
+        self.reshape_type, self.dims, self.P, self.me, self.mode = reshape_type, tuple(dims), comm_size, me, mode
+        self.ndims = len(dims)
+        self.pencils = sub_bricks(reshape_type, dims, comm_size, split)
+        self.in_counts, self.in_starts, self.out_counts, self.out_starts = self.pencils[me]
+        (self.send_rows, self.recv_rows, self.send_counts, self.recv_counts, self.strat, self.pack_free,
+         self.unpack_free) = reshape_neighbor_data(reshape_type, self.ndims, self.pencils, me)
+        self.pack_kernel, self.unpack_kernel = reshape_kernel_types(mode, self.pack_free, self.unpack_free)
+        self.send_displs = [r[4] for r in self.send_rows]
+        self.recv_displs = [r[3] for r in self.recv_rows]
+
+    def run_pack(self, pack):
+        if self.pack_kernel == KERNEL_DUMMY:
+            return
+        if self.pack_kernel == KERNEL_PACK_PIPELINED:
+            for i in range(1, self.P + 1):
+                pack(i)
+        else:
+            pack(0)
+
+    def run_unpack(self, unpack):
+        if self.unpack_kernel == KERNEL_DUMMY:
+            return
+        if self.unpack_kernel == KERNEL_UNPACK_PIPELINED:
+            for i in range(1, self.P + 1):
+                unpack(i)
+        else:
+            unpack(0)

@@ -149,3 +149,69 @@ def test_forward_rows_match_general_rows():
     s = pencil.TransposeShare(pencil.Y_TO_Z, (512 * 4, 2048, 2048), 4, 1, "fused", fixed=(512, 0))
     assert [tuple(r) for r in s.send_rows] == send and [tuple(r) for r in s.recv_rows] == recv
     assert s.send_counts == sc and s.recv_counts == rc
+
+
+# ---- bricks <-> pencils reshapes ----------------------------------------------------------
+def simulate_reshape(rt, dims, P, mode, split=None):
+    nd = len(dims)
+    Gf = np.arange(int(np.prod(dims)), dtype=np.uint64).reshape(dims[::-1]).transpose(*range(nd)[::-1])
+    ax = tuple(range(nd))  # both layouts in the same axis order
+    shares = [pencil.ReshapeShare(rt, dims, P, r, mode, split) for r in range(P)]
+    sends = []
+    for s in shares:
+        inbuf = block(Gf, ax, s.in_counts, s.in_starts)
+        if s.pack_kernel == pencil.KERNEL_DUMMY:
+            sends.append(inbuf)  # pack-free: the chunks already lie at their displacements
+            continue
+        buf = np.zeros(max(1, sum(s.send_counts)), dtype=np.uint64)
+        rows = np.array(s.send_rows)
+        s.run_pack(lambda i, s=s, inbuf=inbuf, buf=buf, rows=rows: O.host_kernel(
+            s.pack_kernel, s.in_counts, inbuf, buf, rows, i if i else None))
+        sends.append(buf)
+    for r, s in enumerate(shares):
+        recv = np.zeros(max(1, sum(s.recv_counts)), dtype=np.uint64)
+        for i, t in enumerate(shares):
+            n = t.send_counts[r]
+            assert n == s.recv_counts[i]
+            recv[s.recv_displs[i]:s.recv_displs[i] + n] = sends[i][t.send_displs[r]:t.send_displs[r] + n]
+        want = block(Gf, ax, s.out_counts, s.out_starts)
+        if s.unpack_kernel == pencil.KERNEL_DUMMY:
+            out = recv[:want.size]  # unpack-free: the chunks land in the final layout
+        else:
+            out = np.zeros(int(np.prod(s.out_counts)), dtype=np.uint64)
+            rows = np.array(s.recv_rows)
+            s.run_unpack(lambda i, s=s, recv=recv, out=out, rows=rows: O.host_kernel(
+                s.unpack_kernel, s.out_counts, recv, out, rows, i if i else None))
+        assert np.array_equal(out, want), (rt, dims, P, mode, split, r, s.strat)
+    return shares
+
+
+RESHAPES = [pencil.X_BRICKS_TO_PENCILS, pencil.X_PENCILS_TO_BRICKS, pencil.Z_PENCILS_TO_BRICKS,
+            pencil.Z_BRICKS_TO_PENCILS]
+
+
+@pytest.mark.parametrize("mode", pencil.RESHAPE_MODES)
+@pytest.mark.parametrize("rt", RESHAPES)
+@pytest.mark.parametrize("dims,P,split,strat", [
+    ((17, 10, 7), 3, (1, 3), 1),    # pencils split along the third axis only: pack/unpack-free
+    ((9, 23, 13), 4, (4, 1), 2),    # along the second axis only
+    ((12, 11, 9), 4, (2, 2), 3),    # along both
+    ((8, 8, 8), 6, (3, 2), 3),
+    ((5, 3, 11), 4, (2, 2), 3),     # a rank with nothing on one axis
+])
+def test_reshape_3d_equals_global_layout(rt, dims, P, split, strat, mode):
+    """All ranks of a reshape group simulated with the oracle's host kernels (pack, all-to-all,
+    unpack, or none of the copies where the strategy makes them unnecessary) must rebuild each
+    rank's target layout of the global array."""
+    shares = simulate_reshape(rt, dims, P, mode, split)
+    assert all(s.strat == strat for s in shares)
+    b2p = rt in pencil.BRICKS_TO_PENCILS
+    assert all(s.pack_free == (b2p and strat == 1) for s in shares)
+    assert all(s.unpack_free == ((not b2p) and strat == 1) for s in shares)
+
+
+@pytest.mark.parametrize("mode", pencil.RESHAPE_MODES)
+@pytest.mark.parametrize("rt", RESHAPES)
+@pytest.mark.parametrize("dims,P", [((18, 155), 4), ((90, 57), 3)])
+def test_reshape_2d_equals_global_layout(rt, dims, P, mode):
+    simulate_reshape(rt, dims, P, mode)
