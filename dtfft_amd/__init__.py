@@ -395,7 +395,7 @@ class UnpackFFT:
 class ExchangeShare:
     """One GPU's share of a forward transpose on P peers (equal splits), with a loopback
     copy in place of the exchange: PACK_FORWARD per peer into contiguous per-peer send
-    chunks, device copy of the send buffer into the receive buffer, then either UNPACK +
+    chunks, a copy kernel moving the send buffer into the receive buffer, then either UNPACK +
     FFT along the aligned axis (``fused=False``) or the fused unpack+FFT (``fused=True``).
     Pencil A = [a, b_me, c] -> B = [b, c, a_me]."""
 
@@ -412,6 +412,10 @@ class ExchangeShare:
         bme, ame = b // peers, a // peers
         self.a_dims, self.b_dims = (a, bme, c), (b, c, ame)
         self.pack = Kernel(KernelType.PACK_FORWARD, self.a_dims, self.es, send, effort)
+        # the loopback in place of the exchange: the library's copy kernel (16-byte lanes) over the
+        # whole send buffer, one box of sum(counts) elements
+        n_send = sum(sc)
+        self.loopback = Kernel(KernelType.PACK, [n_send, 1], self.es, [[n_send, 1, 1, 0, 0]], effort)
         self.fused = fused
         if fused:
             self.unpack_fft = UnpackFFT(self.b_dims, recv, precision, effort)
@@ -422,8 +426,7 @@ class ExchangeShare:
     def run(self, A, send, recv, B, sign: int = FFT_FORWARD, tmp=None) -> None:
         for i in range(1, self.P + 1):
             self.pack.execute(A, send, neighbor=i)
-        call("dtfft_hip_memcpy_d2d_async", C.c_void_p(_ptr(recv)), C.c_void_p(_ptr(send)),
-             C.c_size_t(self.bytes), None)
+        self.loopback.execute(send, recv)
         if self.fused:
             self.unpack_fft.execute(recv, B, sign)
         else:
