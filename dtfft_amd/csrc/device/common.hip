@@ -57,4 +57,16 @@ __device__ __forceinline__ cx<double> twmul(cx<double> a, const cx<double> w) {
     return mk(re, im);
 }
 
+// Non-temporal store of one complex value (streaming output that no later instruction of the
+// kernel reads back).  Measured on MI355X at 512^3 FP32: the LDS-DMA column-in / row-out pass
+// went from 430 to 422 us with its row stores non-temporal, the row passes from 377-395 to
+// 365-376 us; column-in passes of the plain kernel and column-out stores did not gain.
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void store_nt(cx<float>* p, cx<float> v) {
+    __builtin_nontemporal_store(*(const u64*)&v, (u64*)p);
+}
+__device__ __forceinline__ void store_nt(cx<double>* p, cx<double> v) {
+    __builtin_nontemporal_store(*(const u64x2*)&v, (u64x2*)p);
+}
+
 #endif
