@@ -193,7 +193,7 @@ class Kernel:
     (nxx, nyy, nzz, din, dout) rows, one per peer."""
 
     def __init__(self, kernel_type, dims, base_storage: int, neighbor_data=None,
-                 effort: Effort = Effort.ESTIMATE):
+                 effort: Effort = Effort.ESTIMATE, force_effort: bool = False):
         self.h = C.c_void_p()
         nd = None
         npeers = 0
@@ -202,7 +202,7 @@ class Kernel:
             npeers = len(rows)
             nd = _i32arr([x for r in rows for x in r])
         call("dtfft_hip_kernel_create", int(kernel_type), len(dims), _i32arr(dims), int(base_storage), nd,
-             npeers, int(effort), 0, C.byref(self.h))
+             npeers, int(effort), int(bool(force_effort)), C.byref(self.h))
 
     def execute(self, inp, out, neighbor: int = 0, stream: int | None = None, sync: bool = False) -> None:
         call("dtfft_hip_kernel_execute", self.h, C.c_void_p(_ptr(inp)), C.c_void_p(_ptr(out)),

@@ -225,6 +225,6 @@ struct mover {
     int32_t init(int es, bool transpose, int tile, int rows, int pad, int vec);
     int32_t run(const box_desc& d, const void* in, void* out, hipStream_t s) const;
 };
-int32_t tune_mover(const box_move& m, int effort, hipStream_t stream, mover* out);
+int32_t tune_mover(const box_move& m, int effort, bool force_effort, hipStream_t stream, mover* out);
 
 }  // namespace dtfft_hip

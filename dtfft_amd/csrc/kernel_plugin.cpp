@@ -15,6 +15,8 @@
 // (routines :757-1021) are supported too.
 #include <algorithm>
 #include <array>
+#include <cstdio>
+#include <cstdlib>
 
 #include "internal.h"
 
@@ -101,7 +103,7 @@ static double time_mover(const mover& mv, const box_desc& d, void* in, void* out
     return ms / iters;
 }
 
-int32_t tune_mover(const box_move& m, int effort, hipStream_t stream, mover* out) {
+int32_t tune_mover(const box_move& m, int effort, bool force_effort, hipStream_t stream, mover* out) {
     int tile, rows, pad, vec;
     default_shape(m.es, m.transpose, tile, rows, pad, vec);
     const auto& d = m.d;
@@ -109,9 +111,13 @@ int32_t tune_mover(const box_move& m, int effort, hipStream_t stream, mover* out
                             "|" + std::to_string(d.e0) + "x" + std::to_string(d.e1) + "x" + std::to_string(d.e2) +
                             "|" + std::to_string(d.is1) + "," + std::to_string(d.os0);
     std::string rec;
-    if (m.transpose && tune_lookup(key, rec)) {
+    // A forced ESTIMATE takes the default shape even where a tuning record exists, as
+    // get_kernel does with force_effort (src/dtfft_kernel_device.F90:326-335).
+    const bool estimate_only = force_effort && effort == DTFFT_HIP_ESTIMATE;
+    const char* source = "default";
+    if (m.transpose && !estimate_only && tune_lookup(key, rec)) {
         int t, r, p;
-        if (std::sscanf(rec.c_str(), "%d %d %d", &t, &r, &p) == 3) { tile = t; rows = r; pad = p; }
+        if (std::sscanf(rec.c_str(), "%d %d %d", &t, &r, &p) == 3) { tile = t; rows = r; pad = p; source = "record"; }
     } else if (m.transpose && effort >= DTFFT_HIP_MEASURE && !compile_only()) {
         // time candidate shapes on scratch buffers of the box's extent
         const int64_t span_in = d.din + (d.e0 - 1) + (d.e1 - 1) * d.is1 + (d.e2 - 1) * d.is2 + 1;
@@ -140,7 +146,10 @@ int32_t tune_mover(const box_move& m, int effort, hipStream_t stream, mover* out
         char buf[64];
         std::snprintf(buf, sizeof buf, "%d %d %d", tile, rows, pad);
         tune_store(key, buf);
+        source = "timed";
     }
+    if (m.transpose && std::getenv("DTFFT_HIP_DEBUG"))
+        std::fprintf(stderr, "[dtfft_hip] %s: tile %d rows %d pad %d (%s)\n", key.c_str(), tile, rows, pad, source);
     return out->init(m.es, m.transpose, tile, rows, pad, m.transpose ? 1 : vec);
 }
 
@@ -247,7 +256,6 @@ extern "C" {
 int32_t dtfft_hip_kernel_create(int32_t kernel_type, int32_t ndims, const int32_t* dims, int64_t base_storage,
                                 const int32_t* neighbor_data, int32_t n_peers, int32_t effort,
                                 int32_t force_effort, void** handle) {
-    (void)force_effort;
     if (!handle || !dims) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
     *handle = nullptr;
     if (ndims != 2 && ndims != 3) return DTFFT_HIP_ERROR_INVALID_N_DIMENSIONS;
@@ -298,7 +306,7 @@ int32_t dtfft_hip_kernel_create(int32_t kernel_type, int32_t ndims, const int32_
         int32_t rc = make_box(kt, ndims, h->dims, needs_neighbors(kt) ? h->nd[p].data() : nullptr, m);
         if (rc) return rc;
         mover mv;
-        rc = tune_mover(m, effort, s, &mv);
+        rc = tune_mover(m, effort, force_effort != 0, s, &mv);
         if (rc) return rc;
         h->boxes.push_back(m);
         h->movers.push_back(mv);

@@ -150,7 +150,9 @@ int32_t dtfft_hip_set_compile_only(int32_t on);
  * permute kernels.  base_storage is the element size in bytes (4, 8 or 16).
  * Follows abstract_kernel%create (src/dtfft_abstract_kernel.F90:219-288): zero dims
  * make a dummy kernel, 2D BACKWARD kernels are remapped to FORWARD ones.
- * effort >= MEASURE times launch configurations on the device (autotune). */
+ * effort >= MEASURE times launch configurations on the device (autotune); a stored
+ * tuning record is used otherwise.  force_effort != 0 with ESTIMATE takes the default
+ * launch shape and ignores tuning records (src/dtfft_kernel_device.F90:326-335). */
 int32_t dtfft_hip_kernel_create(int32_t kernel_type, int32_t ndims, const int32_t* dims,
                                 int64_t base_storage, const int32_t* neighbor_data, int32_t n_peers,
                                 int32_t effort, int32_t force_effort, void** handle);

@@ -111,3 +111,27 @@ def test_compile_only_plan_and_kernels(D, tmp_path):
     with pytest.raises(D.DtfftError) as e:
         p.execute(1, 2, D.Execute.FORWARD)
     assert e.value.code == 606
+
+
+def test_force_effort_estimate_ignores_tuning_records(D, tmp_path, capfd, monkeypatch):
+    """A tuning record for a permute is honoured at ESTIMATE, and ignored when the effort is
+    forced (get_kernel, src/dtfft_kernel_device.F90:326-335): the default shape is taken."""
+    L = D.lib()
+    if D.device_arch().startswith("gfx950"):
+        pytest.skip("a GPU is present; compile-only behaviour is exercised on CPU hosts")
+    dims = [24, 20, 12]
+    key = "mover|T|8|24x20x12|24,240"   # PERMUTE_FORWARD of 8-byte elements: is1 = nx, os0 = ny*nz
+    (tmp_path / "tuning.txt").write_text(f"{key}\t16 4 0\n")
+    default_dir = os.environ.get("DTFFT_HIP_CACHE_DIR") or os.path.join(os.path.dirname(D._lib.LIB_PATH), "kcache")
+    monkeypatch.setenv("DTFFT_HIP_DEBUG", "1")
+    assert L.dtfft_hip_set_cache_dir(str(tmp_path).encode()) == 0
+    try:
+        capfd.readouterr()
+        D.Kernel(D.KernelType.PERMUTE_FORWARD, dims, 8)
+        err = capfd.readouterr().err
+        assert f"{key}: tile 16 rows 4 pad 0 (record)" in err, err
+        D.Kernel(D.KernelType.PERMUTE_FORWARD, dims, 8, force_effort=True)
+        err = capfd.readouterr().err
+        assert f"{key}: tile 32 rows 8 pad 1 (default)" in err, err
+    finally:
+        L.dtfft_hip_set_cache_dir(default_dir.encode())
