@@ -171,14 +171,16 @@ def device_arch() -> str:
 
 def set_config(enable_z_slab: bool = False, enable_fused: bool = True, enable_kernel_autotune: bool = False,
                measure_warmup_iters: int = 2, measure_iters: int = 5, stream: int | None = None,
-               lds_dma: int = 1, twl: int = 1) -> None:
+               lds_dma: int = 1, twl: int = 1, blocked: bool = True) -> None:
     """dtfft_set_config for this library (src/dtfft_config.F90).  ``lds_dma``: 0 never, 1 where
     picked (default), 2 the double-buffer kernel wherever allowed, 3 the single-buffer kernel
     wherever allowed -- column-in passes through LDS tiles filled by direct loads.  ``twl``:
     FFT kernels with the twiddle table in LDS (persistent blocks): 0 never, 1 by the default
-    rule, 2 wherever the kernel has that form and the LDS budget allows it."""
+    rule, 2 wherever the kernel has that form and the LDS budget allows it.  ``blocked``: a row
+    pass feeding a column-in pass writes its output chunk-blocked (private layout) so that the
+    column pass reads contiguous tiles; False keeps the natural layout."""
     c = Config(int(enable_z_slab), int(enable_fused), int(enable_kernel_autotune), measure_warmup_iters,
-               measure_iters, stream, int(lds_dma), int(twl))
+               measure_iters, stream, int(lds_dma), int(twl), int(bool(blocked)))
     call("dtfft_hip_set_config", C.byref(c))
 
 

@@ -32,6 +32,7 @@ class Config(C.Structure):
         ("stream", C.c_void_p),
         ("lds_dma", C.c_int32),
         ("twl", C.c_int32),
+        ("blocked", C.c_int32),
     ]
 
 

@@ -209,15 +209,26 @@ struct lines_desc {
     i64 ntl;             // ceil(nl / LPB)
     i64 ntiles;          // ntl * nb
     i64 in_seg_stride;   // SEG > 0: input element n sits at (n / SEG) * in_seg_stride + (n % SEG) * in_ns
+    // Column-in tiles: input tile tl starts at tl * in_ts (0: tl * LPB * in_ls, lines side by
+    // side).  A chunk-blocked input, where the LPB lines of a tile are stored as one contiguous
+    // block of N rows of LPB elements (in_ns = LPB), has in_ts = N * LPB.
+    i64 in_ts;
+    i64 out_seg_stride;  // OSEG > 0: output element n sits at (n / OSEG) * out_seg_stride + (n % OSEG) * out_ns
 };
 
 // SEG > 0 reads the FFT axis in segments of SEG elements that lie in_seg_stride apart:
 // the per-peer chunks of a receive buffer, so that unpack and FFT are one pass.
-template <typename T, class S, int TPL, int LPB, bool IN_COLS, bool OUT_COLS, int PAD, int SEG = 0>
+// OSEG > 0 writes the FFT axis in segments of OSEG elements that lie out_seg_stride apart:
+// the chunk-blocked layout that a following column-in pass reads as contiguous tiles.
+template <typename T, class S, int TPL, int LPB, bool IN_COLS, bool OUT_COLS, int PAD, int SEG = 0, int OSEG = 0>
 struct fft_tile {
     static __device__ __forceinline__ i64 in_elem(int n, const lines_desc& d) {
         if constexpr (SEG > 0) return (i64)(n / SEG) * d.in_seg_stride + (i64)(n % SEG) * d.in_ns;
         else return (i64)n * d.in_ns;
+    }
+    static __device__ __forceinline__ i64 out_elem(int n, const lines_desc& d) {
+        if constexpr (OSEG > 0) return (i64)(n / OSEG) * d.out_seg_stride + (i64)(n % OSEG) * d.out_ns;
+        else return (i64)n * d.out_ns;
     }
     typedef typename tw_of<T>::type tw_t;
     static constexpr int N = S::N;
@@ -301,7 +312,7 @@ struct fft_tile {
                     if (DST_G) {
                         if (lv) {
                             cx<T> y = v[it][s];
-                            cx<T>* o = &gout[obase + (i64)line * d.out_ls + (i64)n * d.out_ns];
+                            cx<T>* o = &gout[obase + (i64)line * d.out_ls + out_elem(n, d)];
                             // rows in and rows out: non-temporal stores (measured faster on MI355X
                             // for the row passes, slower for column-in and column-out passes)
                             if constexpr (!COLS && !IN_COLS) store_nt(o, inv ? conj(y) : y);
@@ -376,7 +387,7 @@ struct fft_tile {
         const i64 tl = tile % d.ntl;
         const i64 b = tile / d.ntl;
         l0 = tl * LPB;
-        ibase = l0 * d.in_ls + b * d.in_bs;
+        ibase = (d.in_ts ? tl * d.in_ts : l0 * d.in_ls) + b * d.in_bs;
         obase = l0 * d.out_ls + b * d.out_bs;
     }
 
@@ -416,11 +427,12 @@ __device__ __forceinline__ i64 xcd_tile(i64 slot, i64 G) {
 }
 }  // namespace dtfft
 
-template <typename T, class S, int TPL, int LPB, bool IN_COLS, bool OUT_COLS, int PAD, int SEG = 0, bool TWL = false>
+template <typename T, class S, int TPL, int LPB, bool IN_COLS, bool OUT_COLS, int PAD, int SEG = 0, bool TWL = false,
+          int OSEG = 0>
 __global__ void __launch_bounds__(TPL * LPB)
 fft_c2c_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
                const typename tw_of<T>::type* __restrict__ tw, dtfft::lines_desc d, int inv) {
-    typedef dtfft::fft_tile<T, S, TPL, LPB, IN_COLS, OUT_COLS, PAD, SEG> F;
+    typedef dtfft::fft_tile<T, S, TPL, LPB, IN_COLS, OUT_COLS, PAD, SEG, OSEG> F;
     typedef typename tw_of<T>::type tw_t;
     __shared__ cx<T> lds[F::LDS_ELEMS];
     const tw_t* twp = tw;
@@ -736,7 +748,7 @@ struct cols_dma {
 
     static __device__ __forceinline__ void bases(const lines_desc& d, i64 t, i64& ibase, i64& obase) {
         const i64 tl = t % d.ntl, b = t / d.ntl;
-        ibase = tl * LPB + b * d.in_bs;
+        ibase = tl * (d.in_ts ? d.in_ts : (i64)LPB) + b * d.in_bs;
         obase = tl * LPB * d.out_ls + b * d.out_bs;
     }
 

@@ -71,8 +71,9 @@ std::string fft_config::expr() const {
     s += ">, " + std::to_string(tpl) + ", " + std::to_string(lpb) + ", ";
     if (kind == 0) s += std::string(in_cols ? "true" : "false") + ", " + (out_cols ? "true" : "false") + ", ";
     s += std::to_string(pad);
-    if (kind == 0 && (seg > 0 || twl)) s += ", " + std::to_string(seg);
-    if (kind == 0 && twl) s += ", true";
+    if (kind == 0 && (seg > 0 || twl || oseg > 0)) s += ", " + std::to_string(seg);
+    if (kind == 0 && (twl || oseg > 0)) s += twl ? ", true" : ", false";
+    if (kind == 0 && oseg > 0) s += ", " + std::to_string(oseg);
     if (kind == 2 && (in_cols || twl)) s += in_cols ? ", true" : ", false";
     if (kind == 2 && twl) s += ", true";
     s += ">";

@@ -74,6 +74,8 @@ struct lines_desc {  // mirrors dtfft::lines_desc in device/fft.hip
     int64_t out_ls, out_bs, out_ns;
     int64_t ntl, ntiles;
     int64_t in_seg_stride;  // with fft_config::seg > 0: input chunks of `seg` elements this far apart
+    int64_t in_ts;           // column-in tile stride of a chunk-blocked input (0: lines side by side)
+    int64_t out_seg_stride;  // with fft_config::oseg > 0: output chunks of `oseg` elements this far apart
 };
 
 // Launch configuration of one batched 1D FFT kernel.
@@ -90,6 +92,7 @@ struct fft_config {
     int dma = 0;  // column-in passes through LDS filled by direct loads: 1 double buffer
                   // (fft_cols_dma_kernel), 2 single buffer, two blocks per CU (fft_cols_dma1_kernel)
     int twl = 0;  // C2C (kind 0, dma 0): persistent blocks with the twiddle table in LDS
+    int oseg = 0;  // C2C rows (kind 0, dma 0): FFT axis written in chunks of `oseg` elements
     // fft_any_kernel (device/anylen.hip): any == 1 runs a line of logical length nlog through
     // the n-point schedule in LDS, directly (chirp false, n == nlog) or by Bluestein's chirp
     // z-transform (chirp true, n >= 2 nlog - 1); kind 0 / 1 / 2 = C2C / R2C / C2R of nlog

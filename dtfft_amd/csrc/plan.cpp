@@ -25,7 +25,7 @@ extern "C" int32_t dtfft_hip_kernel_destroy(void*);
 
 namespace dtfft_hip {
 
-static dtfft_hip_config_t g_config = {0, 1, 0, 2, 5, nullptr, 1, 1};
+static dtfft_hip_config_t g_config = {0, 1, 0, 2, 5, nullptr, 1, 1, 1};
 
 const dtfft_hip_config_t& global_config() { return g_config; }
 
@@ -63,6 +63,10 @@ struct plan_t {
     fft_stage fx, fx_c2r, fy, fz, fy_cols;           // plain stages
     fft_stage u_xy, u_yz, u_zx, u_yx;                // fused stages (u_yx: 2D only)
     fft_stage fy_zc, u_zx_c2r, u_yx_c2r;             // fused R2C backward (3D, 3D, 2D)
+    // chunk-blocked pairs (row pass -> column-in pass) of the fused C2C sequences:
+    // forward fx -> u_xy, backward fz -> u_zx (3D) / fy -> u_yx (2D)
+    fft_stage fx_blk, u_xy_blk, fb_blk, ub_blk;
+    bool blk_fwd = false, blk_bwd = false;
     bool r2r = false;
     int kind[3] = {0, 0, 0};                         // R2R kinds along X, Y, Z
     r2r_stage rx, ry, rz, rxb, ryb, rzb;             // R2R forward / backward stages
@@ -110,6 +114,54 @@ static int32_t fused(plan_t* p, int64_t n, int64_t nl, int64_t nb, int64_t in_bs
                           st, p->lds_dma);
 }
 
+// Chunk-blocked variant of a (row pass, column-in pass) pair.  `cols` reads the natural output
+// of `rows`: nl = rows' length lines side by side (in_ls 1), its FFT axis rows' lines
+// (in_ns = rows' length), its batches whole planes of rows' output.  The blocked pair writes
+// each plane as chunks of B = cols' lpb lines: element (line x, row y) of a plane at
+// (x / B) * (ny * B) + y * B + x % B, ny = cols' FFT length.  A column tile is then one
+// contiguous block of ny * B elements (B * ny * 8 bytes, 64 KiB at 512 FP32 points), which the
+// LDS-DMA column kernel streams instead of gathering 128-byte pieces of ny rows; the row pass
+// writes B-element pieces of its lines, each continuing the previous line's piece (runs of
+// LPB * B elements per block).  The pair is built where cols runs a column-in kernel (LDS-DMA,
+// whose tile row is B elements = 128 bytes, or the plain one) and B divides its lines.
+static int32_t blocked_pair(plan_t* p, const fft_stage& rows, const fft_stage& cols, fft_stage* rows_blk,
+                            fft_stage* cols_blk, bool* ok) {
+    *ok = false;
+    const fft_config& rc = rows.cfg;
+    const fft_config& cc = cols.cfg;
+    const int64_t es = (int64_t)p->cplx;
+    if (!global_config().blocked || !rows.k || !cols.k) return DTFFT_HIP_SUCCESS;
+    if (rc.kind != 0 || rc.any || rc.dma || rc.seg || rc.in_cols || rc.out_cols || rc.n <= 1) return DTFFT_HIP_SUCCESS;
+    if (cc.kind != 0 || cc.any || !cc.in_cols || cc.out_cols || cc.seg || cc.n <= 1) return DTFFT_HIP_SUCCESS;
+    const int64_t B = cc.lpb, nrow = rc.n, ny = cc.n;
+    const lines_desc& cd = cols.d;
+    if ((cc.dma && B * es != 128) || cd.nl != nrow || cd.nl % B != 0 || cd.in_ls != 1 || cd.in_ns != nrow) return DTFFT_HIP_SUCCESS;
+    if (cd.nb > 1 && cd.in_bs != nrow * ny) return DTFFT_HIP_SUCCESS;
+    if (rows.d.nl * rows.d.nb != ny * cd.nb || rows.d.in_ls != nrow || rows.d.in_ns != 1) return DTFFT_HIP_SUCCESS;
+    fft_config r = rc;
+    r.oseg = (int)B;
+    lines_desc rd = make_lines(ny, cd.nb, nrow, nrow * ny, 1, B, nrow * ny, 1, r.lpb);
+    rd.out_seg_stride = ny * B;
+    lines_desc bd = cd;
+    bd.in_ns = B;
+    bd.in_ts = ny * B;
+    if (cc.dma && !dma_ok(cc, bd)) return DTFFT_HIP_SUCCESS;
+    // On the blocked input the persistent double-buffer kernel is the faster LDS-DMA form:
+    // measured on MI355X at 512^3 FP32, 386-395 us per pass against 397-401 us with one tile
+    // per block (and 414-418 us for the plain column kernel); on gathered input it is the
+    // other way round.  Under the default rule (lds_dma 1) the blocked stage takes it.
+    fft_config c = cc;
+    if (c.dma == 2 && p->lds_dma == 1) {
+        fft_config c2 = c;
+        c2.dma = 1;
+        if (dma_ok(c2, bd)) c = c2;
+    }
+    TRY(rows_blk->init(r, rd));
+    TRY(cols_blk->init(c, bd));
+    *ok = true;
+    return DTFFT_HIP_SUCCESS;
+}
+
 // R2R stages along X, Y (and Z), each over the rows of its pencil; the backward stages
 // use the inverse kinds (src/dtfft_utils.F90:447-471).
 static int32_t build_r2r(plan_t* p) {
@@ -154,6 +206,10 @@ static int32_t build(plan_t* p) {
                 TRY(plan_c2r_cols_stage((int)nx, p->dbl, ny, 1, 0, ny, nx / 2, 0, &p->u_yx_c2r, p->lds_dma));
             else
                 TRY(fused(p, nx, ny, 1, 0, ny, nx, 0, &p->u_yx));   // Y (ny,nx) -> X (nx,ny)
+            if (!p->r2c) {
+                TRY(blocked_pair(p, p->fx, p->u_xy, &p->fx_blk, &p->u_xy_blk, &p->blk_fwd));
+                TRY(blocked_pair(p, p->fy, p->u_yx, &p->fb_blk, &p->ub_blk, &p->blk_bwd));
+            }
         }
         return DTFFT_HIP_SUCCESS;
     }
@@ -197,6 +253,8 @@ static int32_t build(plan_t* p) {
                                     p->lds_dma));
         } else {
             TRY(fused(p, nx, nz, ny, nz * nc, nz, nc * ny, nc, &p->u_zx));  // Z -> X + FFT_X
+            TRY(blocked_pair(p, p->fx, p->u_xy, &p->fx_blk, &p->u_xy_blk, &p->blk_fwd));
+            TRY(blocked_pair(p, p->fz, p->u_zx, &p->fb_blk, &p->ub_blk, &p->blk_bwd));
         }
     }
     return DTFFT_HIP_SUCCESS;
@@ -367,28 +425,34 @@ static int32_t exec_fused(plan_t* p, void* in, void* out, bool fwd, void* aux, s
     hipStream_t s = p->stream;
     const int F = DTFFT_HIP_FFT_FORWARD, B = DTFFT_HIP_FFT_BACKWARD;
     TRY(ensure_aux(p, aux));
+    // the chunk-blocked pairs write their intermediate into aux, or into out when out is not
+    // the input (the forward 3D row pass writes out, which the column pass then reads)
     if (p->ndims == 2) {
         if (fwd) {
-            STEP(r, p->fx.run(in, aux, F, s));
-            STEP(r, p->u_xy.run(aux, out, F, s));
+            const bool blk = p->blk_fwd;
+            STEP(r, (blk ? p->fx_blk : p->fx).run(in, aux, F, s));
+            STEP(r, (blk ? p->u_xy_blk : p->u_xy).run(aux, out, F, s));
         } else {
-            STEP(r, p->fy.run(in, aux, B, s));
-            STEP(r, (p->r2c ? p->u_yx_c2r : p->u_yx).run(aux, out, B, s));
+            const bool blk = p->blk_bwd && !p->r2c;
+            STEP(r, (blk ? p->fb_blk : p->fy).run(in, aux, B, s));
+            STEP(r, (p->r2c ? p->u_yx_c2r : blk ? p->ub_blk : p->u_yx).run(aux, out, B, s));
         }
         return DTFFT_HIP_SUCCESS;
     }
     // 3D, generic or Z-slab (same Z-pencil output on one rank)
     if (fwd) {
-        STEP(r, p->fx.run(in, out, F, s));
-        STEP(r, p->u_xy.run(out, aux, F, s));
+        const bool blk = p->blk_fwd && in != out;
+        STEP(r, (blk ? p->fx_blk : p->fx).run(in, out, F, s));
+        STEP(r, (blk ? p->u_xy_blk : p->u_xy).run(out, aux, F, s));
         STEP(r, p->u_yz.run(aux, out, F, s));
     } else if (p->r2c) {
         STEP(r, p->fz.run(in, aux, B, s));
         STEP(r, p->fy_zc.run(aux, aux, B, s));
         STEP(r, p->u_zx_c2r.run(aux, out, B, s));
     } else {
-        STEP(r, p->fz.run(in, aux, B, s));
-        STEP(r, p->u_zx.run(aux, out, B, s));
+        const bool blk = p->blk_bwd;
+        STEP(r, (blk ? p->fb_blk : p->fz).run(in, aux, B, s));
+        STEP(r, (blk ? p->ub_blk : p->u_zx).run(aux, out, B, s));
         STEP(r, p->fy_cols.run(out, out, B, s));
     }
     return DTFFT_HIP_SUCCESS;
@@ -408,7 +472,7 @@ extern "C" {
 
 int32_t dtfft_hip_create_config(dtfft_hip_config_t* c) {
     if (!c) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
-    *c = dtfft_hip_config_t{0, 1, 0, 2, 5, nullptr, 1, 1};
+    *c = dtfft_hip_config_t{0, 1, 0, 2, 5, nullptr, 1, 1, 1};
     return DTFFT_HIP_SUCCESS;
 }
 
@@ -418,6 +482,7 @@ int32_t dtfft_hip_set_config(const dtfft_hip_config_t* c) {
     if (c->measure_iters <= 0) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
     if (c->lds_dma < 0 || c->lds_dma > 3) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
     if (c->twl < 0 || c->twl > 2) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
+    if (c->blocked < 0 || c->blocked > 1) return DTFFT_HIP_ERROR_INVALID_ARGUMENT;
     g_config = *c;
     return DTFFT_HIP_SUCCESS;
 }
@@ -643,6 +708,8 @@ int32_t dtfft_hip_report(dtfft_hip_plan_t plan) {
     show("fft_x", p->fx); show("fft_y", p->fy); show("fft_z", p->fz); show("fft_ycol", p->fy_cols);
     show("xy+fft", p->u_xy); show("yz+fft", p->u_yz); show("zx+fft", p->u_zx); show("yx+fft", p->u_yx);
     show("fft_yzc", p->fy_zc); show("zx+c2r", p->u_zx_c2r); show("yx+c2r", p->u_yx_c2r);
+    show("fwd_rows", p->fx_blk); show("fwd_cols", p->u_xy_blk); show("bwd_rows", p->fb_blk);
+    show("bwd_cols", p->ub_blk);
     std::fflush(stdout);
     return DTFFT_HIP_SUCCESS;
 }

@@ -210,6 +210,10 @@ typedef struct {
                                      of n >= 2048, FP32 column-in passes without LDS-DMA, the fused
                                      C2R, on stages of >= 2048 tiles), 2 every stage whose kernel has
                                      the form and whose LDS budget allows it */
+    int32_t blocked;              /* 1 (default): a row FFT pass whose output the next pass reads as
+                                     column tiles writes it chunk-blocked (each tile's lines as one
+                                     contiguous block), so that the column pass streams it; the
+                                     intermediate layout is private to the plan.  0: natural layout */
 } dtfft_hip_config_t;
 
 int32_t dtfft_hip_create_config(dtfft_hip_config_t* config);
