@@ -92,3 +92,71 @@ box_transpose(void* __restrict__ vout, const void* __restrict__ vin, box_desc d)
         __syncthreads();
     }
 }
+
+// box_transpose with 16-byte lanes for 4- and 8-byte elements (V = 16 / ES elements per
+// vector): a TILE x TILE tile, (TILE / V) x ROWS threads.  Loads: each lane reads V
+// consecutive elements along i0 (the contiguous input axis) as one 16-byte access and writes
+// them into the LDS tile row i1; stores: each lane gathers V consecutive i1 of one i0 from
+// the tile and writes them as one 16-byte access along i1 (the contiguous output axis).
+// With PAD = 1 the gather of a wave lands on distinct LDS banks (lane stride V (TILE + 1)
+// words).  The host launches it only when every row start on both sides is a multiple of V
+// elements and both pointers are 16-byte aligned; vectors that cross the box edge fall back
+// to per-element moves.
+template <int ES, int V, int TILE, int ROWS, int PAD>
+__global__ void __launch_bounds__((TILE / V) * ROWS)
+box_transpose_v(void* __restrict__ vout, const void* __restrict__ vin, box_desc d) {
+    typedef typename word<ES>::type W;
+    typedef W vec __attribute__((ext_vector_type(V)));
+    static_assert(V * ES == 16 && TILE % V == 0 && TILE % ROWS == 0, "16-byte lanes over whole tiles");
+    constexpr int TX = TILE / V;
+    const W* __restrict__ in = (const W*)vin;
+    W* __restrict__ out = (W*)vout;
+    __shared__ W tile[TILE][TILE + PAD];
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    for (i64 blk = blockIdx.x; blk < d.nblocks; blk += gridDim.x) {
+        const i64 b0 = blk % d.t0;
+        const i64 r = blk / d.t0;
+        const i64 b1 = r % d.t1;
+        const i64 i2 = r / d.t1;
+        const i64 x0 = b0 * TILE, y0 = b1 * TILE;
+        const i64 ib = d.din + i2 * d.is2;
+        const i64 ob = d.dout + i2 * d.os2;
+        // load: tile[i1 - y0][i0 - x0], V elements of a row per lane
+#pragma unroll
+        for (int k = 0; k < TILE; k += ROWS) {
+            const i64 i0 = x0 + tx * V, i1 = y0 + ty + k;
+            if (i1 < d.e1) {
+                const W* src = in + ib + i0 + i1 * d.is1;
+                if (i0 + V <= d.e0) {
+                    const vec w = *(const vec*)src;
+#pragma unroll
+                    for (int v = 0; v < V; v++) tile[ty + k][tx * V + v] = w[v];
+                } else {
+#pragma unroll
+                    for (int v = 0; v < V; v++)
+                        if (i0 + v < d.e0) tile[ty + k][tx * V + v] = src[v];
+                }
+            }
+        }
+        __syncthreads();
+        // store: V consecutive i1 of one i0 per lane
+#pragma unroll
+        for (int k = 0; k < TILE; k += ROWS) {
+            const i64 i0 = x0 + ty + k, i1 = y0 + tx * V;
+            if (i0 < d.e0) {
+                W* dst = out + ob + i1 + i0 * d.os0;
+                if (i1 + V <= d.e1) {
+                    vec w;
+#pragma unroll
+                    for (int v = 0; v < V; v++) w[v] = tile[tx * V + v][ty + k];
+                    *(vec*)dst = w;
+                } else {
+#pragma unroll
+                    for (int v = 0; v < V; v++)
+                        if (i1 + v < d.e1) dst[v] = tile[tx * V + v][ty + k];
+                }
+            }
+        }
+        __syncthreads();
+    }
+}

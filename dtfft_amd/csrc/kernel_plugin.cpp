@@ -23,13 +23,21 @@
 namespace dtfft_hip {
 
 // ---- movers ----------------------------------------------------------------------
+// Fallback tile of the 16-byte-lane transpose, for boxes or pointers not aligned to its vectors.
+static const int kScalarTile = 32, kScalarRows = 8;
+
 int32_t mover::init(int es_, bool transpose_, int tile_, int rows_, int pad_, int vec_) {
     es = es_; transpose = transpose_; tile = tile_; rows = rows_; pad = pad_; vec = vec_;
     std::string expr;
-    if (transpose)
+    if (transpose && vec > 1) {
+        expr = "box_transpose_v<" + std::to_string(es) + ", " + std::to_string(vec) + ", " + std::to_string(tile) +
+               ", " + std::to_string(rows) + ", " + std::to_string(pad) + ">";
+        TRY(rtc_get("transpose", "box_transpose<" + std::to_string(es) + ", " + std::to_string(kScalarTile) + ", " +
+                                     std::to_string(kScalarRows) + ", 1>", &k_scalar));
+    } else if (transpose) {
         expr = "box_transpose<" + std::to_string(es) + ", " + std::to_string(tile) + ", " +
                std::to_string(rows) + ", " + std::to_string(pad) + ">";
-    else {
+    } else {
         expr = "box_copy<" + std::to_string(es) + ", " + std::to_string(vec) + ">";
         TRY(rtc_get("transpose", "box_copy<" + std::to_string(es) + ", 1>", &k_scalar));
     }
@@ -38,16 +46,30 @@ int32_t mover::init(int es_, bool transpose_, int tile_, int rows_, int pad_, in
 
 static bool aligned_for(const box_desc& d, int vec, const void* in, const void* out, int es);
 
+// Rows of both sides of a transposed box start on whole vectors: the input rows along axis
+// 0 (offsets din + i1 is1 + i2 is2), the output rows along axis 1 (dout + i0 os0 + i2 os2).
+static bool aligned_for_transpose(const box_desc& d, int vec, const void* in, const void* out, int es) {
+    auto m = [&](int64_t v) { return v % vec == 0; };
+    return m(d.din) && m(d.dout) && m(d.is1) && m(d.is2) && m(d.os0) && m(d.os2) && d.os1 == 1 &&
+           ((uintptr_t)in % (size_t)(vec * es)) == 0 && ((uintptr_t)out % (size_t)(vec * es)) == 0;
+}
+
 int32_t mover::run(const box_desc& din, const void* in, void* out, hipStream_t s) const {
     box_desc d = din;
     unsigned grid, block;
     if (d.e0 <= 0 || d.e1 <= 0 || d.e2 <= 0) return DTFFT_HIP_SUCCESS;
     if (transpose) {
-        d.t0 = (d.e0 + tile - 1) / tile;
-        d.t1 = (d.e1 + tile - 1) / tile;
+        // 16-byte lanes need every row start on both sides, and both pointers, aligned to
+        // the vector (the output is contiguous along axis 1: os1 == 1)
+        const bool v = vec > 1 && aligned_for_transpose(d, vec, in, out, es);
+        const int t = v || vec == 1 ? tile : kScalarTile;
+        d.t0 = (d.e0 + t - 1) / t;
+        d.t1 = (d.e1 + t - 1) / t;
         d.nblocks = d.t0 * d.t1 * d.e2;
-        block = (unsigned)(tile * rows);
+        block = (unsigned)(v ? (tile / vec) * rows : vec == 1 ? tile * rows : kScalarTile * kScalarRows);
         grid = (unsigned)std::min<int64_t>(d.nblocks, 1 << 22);
+        void* args[] = {&out, &in, &d};
+        return launch(v || vec == 1 ? k : k_scalar, grid, block, s, args);
     } else {
         const int v = aligned_for(d, vec, in, out, es) ? vec : 1;
         const int64_t chunks = (d.e0 + v * 64 - 1) / (v * 64);
@@ -59,8 +81,6 @@ int32_t mover::run(const box_desc& din, const void* in, void* out, hipStream_t s
         void* args[] = {&out, &in, &d};
         return launch(v == vec ? k : k_scalar, grid, block, s, args);
     }
-    void* args[] = {&out, &in, &d};
-    return launch(k, grid, block, s, args);
 }
 
 static bool aligned_for(const box_desc& d, int vec, const void* in, const void* out, int es) {
@@ -70,17 +90,18 @@ static bool aligned_for(const box_desc& d, int vec, const void* in, const void* 
            ((uintptr_t)in % (size_t)(vec * es)) == 0 && ((uintptr_t)out % (size_t)(vec * es)) == 0;
 }
 
-// Default launch shapes; the autotuner may replace them (tune_mover).  Measured on MI355X
-// over 512^3-class permutes: 32x32 tiles win for every element size (8-byte elements: 5.3
-// TB/s against 4.0 for 64x64); 16-byte elements want 16 rows of threads.
+// Default launch shapes; the autotuner may replace them (tune_mover).  Transposes of 4- and
+// 8-byte elements move 16-byte vectors per lane (box_transpose_v) over 64 x 64 tiles; 16-byte
+// elements use 32 x 32 tiles with 16 rows of threads.  Measured on MI355X over 512^3 permutes
+// (forward / backward): 4-byte elements 195 / 203 us with 32 rows of 16-byte lanes against
+// 250 us for the best one-element-per-lane shape (32 x 32, 2 rows); 8-byte elements 380 / 393
+// us with 16 rows against 400 us (32 x 32, 8 rows).
 static void default_shape(int es, bool transpose, int& tile, int& rows, int& pad, int& vec) {
     pad = 1;
     vec = std::max(1, 16 / es);
     if (transpose) {
-        // 32 x 32 tiles; rows of threads per tile as the autotuner picked them on MI355X at
-        // 512^3 (4-byte elements: 2 rows, 247 us per permute against 294 us with 8)
-        tile = 32;
-        rows = es == 16 ? 16 : es == 4 ? 2 : 8;
+        tile = es == 16 ? 32 : 64;
+        rows = es == 4 ? 32 : 16;
     } else {
         tile = rows = 0;
     }
@@ -106,6 +127,7 @@ static double time_mover(const mover& mv, const box_desc& d, void* in, void* out
 int32_t tune_mover(const box_move& m, int effort, bool force_effort, hipStream_t stream, mover* out) {
     int tile, rows, pad, vec;
     default_shape(m.es, m.transpose, tile, rows, pad, vec);
+    int tvec = vec;  // transposes: 16-byte lanes (16 / es) or one element per lane
     const auto& d = m.d;
     const std::string key = std::string("mover|") + (m.transpose ? "T" : "C") + "|" + std::to_string(m.es) +
                             "|" + std::to_string(d.e0) + "x" + std::to_string(d.e1) + "x" + std::to_string(d.e2) +
@@ -116,8 +138,9 @@ int32_t tune_mover(const box_move& m, int effort, bool force_effort, hipStream_t
     const bool estimate_only = force_effort && effort == DTFFT_HIP_ESTIMATE;
     const char* source = "default";
     if (m.transpose && !estimate_only && tune_lookup(key, rec)) {
-        int t, r, p;
-        if (std::sscanf(rec.c_str(), "%d %d %d", &t, &r, &p) == 3) { tile = t; rows = r; pad = p; source = "record"; }
+        int t, r, p, v = 1;
+        const int got = std::sscanf(rec.c_str(), "%d %d %d %d", &t, &r, &p, &v);
+        if (got >= 3 && (v == 1 || v * m.es == 16)) { tile = t; rows = r; pad = p; tvec = v; source = "record"; }
     } else if (m.transpose && effort >= DTFFT_HIP_MEASURE && !compile_only()) {
         // time candidate shapes on scratch buffers of the box's extent
         const int64_t span_in = d.din + (d.e0 - 1) + (d.e1 - 1) * d.is1 + (d.e2 - 1) * d.is2 + 1;
@@ -127,30 +150,39 @@ int32_t tune_mover(const box_move& m, int effort, bool force_effort, hipStream_t
         if (hipMalloc(&bo, (size_t)span_out * m.es) != hipSuccess) { hipFree(bi); return DTFFT_HIP_ERROR_ALLOC_FAILED; }
         double best = 1e30;
         const int tiles[] = {16, 32, 64};
-        const int rowss[] = {1, 2, 4, 8, 16};
+        const int rowss[] = {1, 2, 4, 8, 16, 32};
         const int pads[] = {0, 1, 2};
-        for (int t : tiles)
-            for (int r : rowss)
-                for (int p : pads) {
-                    if (t * r > 1024 || t * r < 64 || r > t) continue;
-                    if ((size_t)t * (t + p) * m.es > 64 * 1024) continue;
-                    if (effort < DTFFT_HIP_EXHAUSTIVE && p == 2) continue;
-                    mover cand;
-                    if (cand.init(m.es, true, t, r, p, 1)) continue;
-                    double ms = time_mover(cand, d, bi, bo, stream, 2, 5);
-                    if (ms < best) { best = ms; tile = t; rows = r; pad = p; }
-                }
+        // one element per lane, and 16-byte lanes for 4- and 8-byte elements where the box's
+        // rows start on whole vectors (the scratch buffers are 16-byte aligned)
+        const int V = 16 / m.es;
+        const bool vec_ok = V > 1 && aligned_for_transpose(d, V, bi, bo, m.es);
+        std::vector<int> vecs = {1};
+        if (vec_ok) vecs.push_back(V);
+        for (int v : vecs)
+            for (int t : tiles)
+                for (int r : rowss)
+                    for (int p : pads) {
+                        const int threads = (t / v) * r;
+                        if (threads > 1024 || threads < 64 || r > t || t % r != 0) continue;
+                        if ((size_t)t * (t + p) * m.es > 64 * 1024) continue;
+                        if (effort < DTFFT_HIP_EXHAUSTIVE && p == 2) continue;
+                        mover cand;
+                        if (cand.init(m.es, true, t, r, p, v)) continue;
+                        double ms = time_mover(cand, d, bi, bo, stream, 2, 5);
+                        if (ms < best) { best = ms; tile = t; rows = r; pad = p; tvec = v; }
+                    }
         hipStreamSynchronize(stream);
         hipFree(bi);
         hipFree(bo);
         char buf[64];
-        std::snprintf(buf, sizeof buf, "%d %d %d", tile, rows, pad);
+        std::snprintf(buf, sizeof buf, "%d %d %d %d", tile, rows, pad, tvec);
         tune_store(key, buf);
         source = "timed";
     }
     if (m.transpose && std::getenv("DTFFT_HIP_DEBUG"))
-        std::fprintf(stderr, "[dtfft_hip] %s: tile %d rows %d pad %d (%s)\n", key.c_str(), tile, rows, pad, source);
-    return out->init(m.es, m.transpose, tile, rows, pad, m.transpose ? 1 : vec);
+        std::fprintf(stderr, "[dtfft_hip] %s: tile %d rows %d pad %d vec %d (%s)\n", key.c_str(), tile, rows, pad,
+                     tvec, source);
+    return out->init(m.es, m.transpose, tile, rows, pad, m.transpose ? tvec : vec);
 }
 
 // ---- kernel type -> box -------------------------------------------------------------

@@ -129,9 +129,9 @@ def test_force_effort_estimate_ignores_tuning_records(D, tmp_path, capfd, monkey
         capfd.readouterr()
         D.Kernel(D.KernelType.PERMUTE_FORWARD, dims, 8)
         err = capfd.readouterr().err
-        assert f"{key}: tile 16 rows 4 pad 0 (record)" in err, err
+        assert f"{key}: tile 16 rows 4 pad 0 vec 1 (record)" in err, err
         D.Kernel(D.KernelType.PERMUTE_FORWARD, dims, 8, force_effort=True)
         err = capfd.readouterr().err
-        assert f"{key}: tile 32 rows 8 pad 1 (default)" in err, err
+        assert f"{key}: tile 64 rows 16 pad 1 vec 2 (default)" in err, err
     finally:
         L.dtfft_hip_set_cache_dir(default_dir.encode())
