@@ -141,15 +141,25 @@ struct r2r_len {
 // tile indices clamped; past the end it re-reads the last line), which keeps the compiler's
 // wait counts exact.  FULL: every tile has LPB lines, so the stores need no guard (a guard
 // becomes a branch, and the compiler then waits for the stores before the next prefetch).
-template <typename T, class S, int TPL, int LPB, int PAD, int KIND, bool FULL>
+//
+// HALF (types 2 and 3, n = 2M even): the n-point complex transform of the real sequence v of
+// Makhoul's map (types 2) or of its Hermitian spectrum V (types 3) runs as an M-point
+// complex transform, halving the butterflies.  Types 2: u_j = v_2j + i v_2j+1 is transformed
+// and untangled, V_k = ((U_k + conj U_{M-k}) - i W^k (U_k - conj U_{M-k})) / 2 for k <= M and
+// V_{n-k} = conj V_k, W = e^{-2 pi i / n} (the R2C untangling of fft_r2c_kernel).  Types 3:
+// Z_k = (V_k + conj V_{M-k}) + i (V_k - conj V_{M-k}) conj(W^k), k < M, whose backward
+// M-point transform is v_2j + i v_2j+1 (the C2R folding of fft_c2r_kernel).  rtwtab holds
+// W^k, k <= M (gen_rtwiddles); it is unused otherwise.
+template <typename T, class S, int TPL, int LPB, int PAD, int KIND, bool FULL, bool HALF = false>
 __global__ void __launch_bounds__(TPL * LPB)
 fft_r2r_kernel(const T* __restrict__ in, T* __restrict__ out, const typename tw_of<T>::type* __restrict__ tw,
-               const cx<T>* __restrict__ t4tab, r2r_desc d) {
+               const cx<T>* __restrict__ t4tab, const typename tw_of<T>::type* __restrict__ rtwtab, r2r_desc d) {
     typedef dtfft::fft_tile<T, S, TPL, LPB, false, false, PAD> F;
     typedef typename tw_of<T>::type tw_t;
     constexpr int M = S::N;
-    constexpr int NX = r2r_len<KIND, M>::n;
+    constexpr int NX = HALF ? 2 * M : r2r_len<KIND, M>::n;
     constexpr bool INV = (KIND == 4 || KIND == 8);  // types 3 run the backward transform
+    static_assert(!HALF || KIND == 4 || KIND == 5 || KIND == 8 || KIND == 9, "half length: types 2 and 3");
     constexpr int XIT = (LPB * NX + F::NT - 1) / F::NT;  // real elements per thread
     constexpr int MIT = (LPB * M + F::NT - 1) / F::NT;   // work elements per thread
     constexpr int TWN = S::TW_SIZE > 0 ? S::TW_SIZE : 1;
@@ -163,6 +173,9 @@ fft_r2r_kernel(const T* __restrict__ in, T* __restrict__ out, const typename tw_
     for (int i = threadIdx.x; i < S::TW_SIZE; i += F::NT) twl[i] = tw[i];
     for (int i = threadIdx.x; i < 2 * NX; i += F::NT) t4l[i] = t4tab[i];
     auto t4 = [&](i64 j) { return t4l[j]; };
+    __shared__ tw_t rtwl[HALF ? M + 1 : 1];
+    if constexpr (HALF)
+        for (int i = threadIdx.x; i <= M; i += F::NT) rtwl[i] = rtwtab[i];
     // element q of this thread in tile t: line (tid + q NT) / NX, position (tid + q NT) % NX
     T pf[XIT];
     auto fetch = [&](i64 t) {
@@ -196,7 +209,22 @@ fft_r2r_kernel(const T* __restrict__ in, T* __restrict__ out, const typename tw_
                 const int line = e / M, m = e % M;
                 const T* x = xs + line * NX;
                 auto xf = [&](i64 j) { return x[j]; };
-                const cx<T> z = r2r_pre_elem<T, KIND>(xf, (i64)NX, (i64)M, (i64)m, t4);
+                cx<T> z;
+                if constexpr (!HALF) {
+                    z = r2r_pre_elem<T, KIND>(xf, (i64)NX, (i64)M, (i64)m, t4);
+                } else if constexpr (INV) {
+                    // V of the full-length map at m and M - m (V_M is real), folded
+                    cx<T> a = r2r_pre_elem<T, KIND>(xf, (i64)NX, (i64)NX, (i64)m, t4);
+                    cx<T> b = r2r_pre_elem<T, KIND>(xf, (i64)NX, (i64)NX, (i64)(M - m), t4);
+                    if (m == 0) { a.y = (T)0; b.y = (T)0; }
+                    b = conj(b);
+                    const cx<T> e2 = a + b;
+                    const cx<T> o2 = twmul(a - b, conj_tw(rtwl[m]));
+                    z = mk(e2.x - o2.y, e2.y + o2.x);
+                } else {
+                    z = mk(r2r_pre_elem<T, KIND>(xf, (i64)NX, (i64)NX, (i64)(2 * m), t4).x,
+                           r2r_pre_elem<T, KIND>(xf, (i64)NX, (i64)NX, (i64)(2 * m + 1), t4).x);
+                }
                 // backward transform = conj(forward(conj z))
                 lds[line * F::LD + F::lidx(m)] = INV ? conj(z) : z;
             }
@@ -212,11 +240,35 @@ fft_r2r_kernel(const T* __restrict__ in, T* __restrict__ out, const typename tw_
                 const i64 li = l0 + line;
                 if (FULL || li < lines) {
                     const cx<T>* W = lds + line * F::LD;
-                    auto wf = [&](i64 j) {
-                        const cx<T> c = W[F::lidx((int)j)];
-                        return INV ? conj(c) : c;
-                    };
-                    out[li * d.out_ls + k] = r2r_post_elem<T, KIND>(wf, (i64)NX, (i64)k, t4);
+                    T y;
+                    if constexpr (!HALF) {
+                        auto wf = [&](i64 j) {
+                            const cx<T> c = W[F::lidx((int)j)];
+                            return INV ? conj(c) : c;
+                        };
+                        y = r2r_post_elem<T, KIND>(wf, (i64)NX, (i64)k, t4);
+                    } else if constexpr (INV) {
+                        // v_j is the real (j even) or imaginary (j odd) part of u_{j / 2}
+                        auto wf = [&](i64 j) {
+                            const cx<T> c = conj(W[F::lidx((int)(j >> 1))]);
+                            return mk((j & 1) ? c.y : c.x, (T)0);
+                        };
+                        y = r2r_post_elem<T, KIND>(wf, (i64)NX, (i64)k, t4);
+                    } else {
+                        // V_k untangled from U, V_{n-k} = conj V_k
+                        auto vf = [&](i64 kf) {
+                            const int kk = kf <= M ? (int)kf : NX - (int)kf;
+                            const cx<T> a = W[F::lidx(kk == M ? 0 : kk)];
+                            const cx<T> b = conj(W[F::lidx(kk == 0 ? 0 : M - kk)]);
+                            const cx<T> s = a + b, df = a - b;
+                            const cx<T> w = twmul(df, rtwl[kk]);
+                            const T half = (T)0.5;
+                            const cx<T> v = mk(half * (s.x + w.y), half * (s.y - w.x));
+                            return kf <= M ? v : conj(v);
+                        };
+                        y = r2r_post_elem<T, KIND>(vf, (i64)NX, (i64)k, t4);
+                    }
+                    out[li * d.out_ls + k] = y;
                 }
             }
         }

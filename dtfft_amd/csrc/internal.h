@@ -179,6 +179,8 @@ struct r2r_stage {
     const rtc_kernel* fk = nullptr;
     std::shared_ptr<twiddles> ftw;  // twiddles of the length-m FFT
     std::shared_ptr<twiddles> t4;   // e^{-i pi j / 4n}, j < 2n
+    std::shared_ptr<twiddles> rtw;  // half-length form: e^{-2 pi i k / n}, k <= n / 2
+    bool half = false;              // fused kernel runs types 2 / 3 through an n/2-point FFT
     unsigned fgrid = 0, fblock = 0;
     int32_t init(int kind, int64_t n, bool dbl, int64_t nl, int64_t nb, int64_t in_ls, int64_t in_bs,
                  int64_t in_es, int64_t out_ls, int64_t out_bs, int64_t out_es, int effort, hipStream_t stream);

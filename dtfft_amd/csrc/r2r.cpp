@@ -24,6 +24,23 @@ int r2r_inverse_kind(int kind) {  // src/dtfft_utils.F90:447-471
     }
 }
 
+// Radix schedule of the half-length form: the power-of-two part in radix-4 passes (one radix 2
+// if the exponent is odd), the odd primes of factorize after it.  With M / 4 threads per line
+// every thread holds one radix-4 butterfly per pass and 8 real elements of the prefetch;
+// factorize's radix-16 passes would leave 16 threads per line holding 32 (293 VGPRs at
+// M = 256 against 117).
+static bool half_schedule(int m, std::vector<int>& out) {
+    std::vector<int> f;
+    if (!factorize(m, f)) return false;
+    int p2 = 0, rest = m;
+    while (rest % 2 == 0) { rest /= 2; p2++; }
+    out.assign((size_t)(p2 / 2), 4);
+    if (p2 % 2) out.push_back(2);
+    for (int r : f) if (r % 2 != 0) out.push_back(r);
+    if (out.empty()) out.push_back(1);
+    return out.size() <= 10;
+}
+
 static int64_t work_length(int kind, int64_t n) {
     if (kind == 3) return 2 * (n - 1);
     if (kind == 7) return 2 * (n + 1);
@@ -58,26 +75,43 @@ int32_t r2r_stage::init(int kind_, int64_t n_, bool dbl_, int64_t nl, int64_t nb
     // line itself: a block reads its lines whole before writing them, so in place is safe.
     // Batches that continue the line spacing fold into one range of lines.
     const bool even = nb == 1 || (in_bs == nl * in_ls && out_bs == nl * out_ls);
-    if (factors && in_es == 1 && out_es == 1 && in_ls == out_ls && in_bs == out_bs && even && m > 1) {
-        fft_config c = default_fft_config((int)m, dbl, false, false, false);
-        {
+    // Types 2 and 3 of even length run the fused kernel's half-length form: the n-point
+    // transform of a real sequence (or of a Hermitian spectrum) as an n/2-point one.
+    std::vector<int> hrad;
+    const bool half_ok = (kind == 4 || kind == 5 || kind == 8 || kind == 9) && n % 2 == 0 && n >= 8 &&
+                         half_schedule((int)(n / 2), hrad);
+    const int64_t mf = half_ok ? n / 2 : m;  // length of the fused kernel's FFT
+    if ((factors || half_ok) && in_es == 1 && out_es == 1 && in_ls == out_ls && in_bs == out_bs && even && m > 1) {
+        fft_config c = default_fft_config((int)mf, dbl, false, false, false);
+        if (half_ok) {
+            // M / 4 threads per line (one butterfly of radix <= 4 per pass), 256-thread blocks
+            c.radices = hrad;
+            const int rmax = *std::max_element(hrad.begin(), hrad.end());
+            c.tpl = (int)std::max<int64_t>(1, mf / std::max(4, rmax));
+            c.lpb = std::max(1, 256 / c.tpl);
+        } else {
             // one butterfly of the largest radix per thread and pass
             const int rmax = *std::max_element(c.radices.begin(), c.radices.end());
-            if ((m / rmax) * c.lpb <= 1024) c.tpl = (int)(m / rmax);
+            if ((mf / rmax) * c.lpb <= 1024) c.tpl = (int)(mf / rmax);
         }
         const size_t ec = dbl ? 16 : 8, er = dbl ? 8 : 4;
-        // work lines, real lines, FFT twiddles (m - 1 of 16 bytes), table (2n complex)
+        // work lines, real lines, FFT twiddles (mf - 1 of 16 bytes), table (2n complex), and
+        // the half-length form's untangling table (n/2 + 1 of 16 bytes)
         const size_t lds = (size_t)c.lpb * (size_t)(c.n + c.pad) * ec + (size_t)c.lpb * n * er +
-                           (size_t)std::max<int64_t>(1, m - 1) * 16 + (size_t)(2 * n) * ec;
+                           (size_t)std::max<int64_t>(1, mf - 1) * 16 + (size_t)(2 * n) * ec +
+                           (half_ok ? (size_t)(n / 2 + 1) * 16 : 16);
         if (lds <= 160 * 1024) {
             d = r2r_desc_h{nl * nb, 1, in_ls, nl * nb * in_ls, 1, out_ls, nl * nb * out_ls, 1, n, m, m};
-            std::string e = "fft_r2r_kernel<" + t + ", dtfft::sched<" + std::to_string(m);
+            std::string e = "fft_r2r_kernel<" + t + ", dtfft::sched<" + std::to_string(mf);
             for (int r : c.radices) e += ", " + std::to_string(r);
             e += ">, " + std::to_string(c.tpl) + ", " + std::to_string(c.lpb) + ", " + std::to_string(c.pad) + ", " +
-                 std::to_string(kind) + ((nl * nb) % c.lpb == 0 ? ", true>" : ", false>");
+                 std::to_string(kind) + ((nl * nb) % c.lpb == 0 ? ", true" : ", false") +
+                 (half_ok ? ", true>" : ">");
             TRY(rtc_get("r2r", e, &fk));
-            TRY(get_twiddles((int)m, c.radices, dbl, &ftw));
+            TRY(get_twiddles((int)mf, c.radices, dbl, &ftw));
             TRY(get_r2r_table(n, dbl, &t4));
+            if (half_ok) TRY(get_rtwiddles((int)n, dbl, &rtw));
+            half = half_ok;
             const int64_t tiles = (nl * nb + c.lpb - 1) / c.lpb;
             fblock = (unsigned)(c.tpl * c.lpb);
             // persistent: as many blocks as are resident
@@ -135,7 +169,8 @@ int32_t r2r_stage::run(const void* in, void* out, void* work, hipStream_t s) con
         r2r_desc_h dd = d;
         void* twp = ftw ? ftw->ptr : nullptr;
         void* tp = t4 ? t4->ptr : nullptr;
-        void* args[] = {(void*)&in, &out, &twp, &tp, &dd};
+        void* rp = rtw ? rtw->ptr : nullptr;
+        void* args[] = {(void*)&in, &out, &twp, &tp, &rp, &dd};
         return launch(fk, fgrid, fblock, s, args);
     }
     if (!work) {

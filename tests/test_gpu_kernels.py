@@ -167,3 +167,39 @@ def test_autotuned_kernel_is_exact():
     k = D.Kernel(KT.PERMUTE_FORWARD, dims, 8, effort=D.Effort.MEASURE)
     k.execute(di, do, sync=True)
     assert same(do.download(inp.size, inp.dtype), want)
+
+
+def run_offset(kt, dims, es, nd, neighbor, in_shift, out_shift, out_size):
+    """run_both with the device pointers moved by whole elements (in_shift / out_shift), so a
+    16-byte-lane kernel sees 4- or 8-byte-aligned pointers and has to take its fallback."""
+    n = int(np.prod(dims))
+    inp = payload(n, es, seed=3)
+    want = np.zeros(out_size, dtype=inp.dtype)
+    O.host_kernel(kt, dims, inp, want, nd, neighbor or None)
+    di = D.DeviceArray.from_numpy(np.concatenate([np.zeros(in_shift, dtype=inp.dtype), inp]))
+    do = D.DeviceArray.from_numpy(np.zeros(out_size + out_shift, dtype=inp.dtype))
+    k = D.Kernel(kt, dims, es, nd)
+    k.execute(di.ptr + in_shift * es, do.ptr + out_shift * es, neighbor=neighbor, sync=True)
+    got = do.download(out_size + out_shift, inp.dtype)[out_shift:]
+    return got, want
+
+
+@pytest.mark.parametrize("es", [4, 8])
+@pytest.mark.parametrize("case", [
+    # (kernel, dims, neighbour rows, neighbour): 16-byte lanes on aligned boxes whose extents
+    # end in part of a vector (input side: nxx = 30; output side: UNPACK_FORWARD with nxx = 30),
+    # and boxes whose offsets or strides are not whole vectors (per-element fallback)
+    (KT.PACK_FORWARD, (96, 40, 12), [[30, 40, 12, 0, 0]], 1),
+    (KT.PACK_FORWARD, (96, 40, 12), [[32, 40, 12, 32, 0], [64, 40, 12, 0, 15360]], 2),
+    (KT.PACK_FORWARD, (96, 40, 12), [[30, 40, 12, 30, 0]], 1),
+    (KT.UNPACK_FORWARD, (96, 40, 12), [[30, 40, 12, 0, 0]], 1),
+    (KT.UNPACK_FORWARD, (96, 40, 12), [[36, 40, 12, 0, 60]], 1),
+    (KT.PERMUTE_FORWARD, (68, 36, 9), None, 0),
+    (KT.PERMUTE_BACKWARD, (36, 9, 68), None, 0),
+])
+def test_vector_lane_edges_and_fallback(case, es):
+    kt, dims, nd, neighbor = case
+    n = int(np.prod(dims))
+    for shift in (0, 1):   # shift 1: pointers off the 16-byte grid for every element size here
+        got, want = run_offset(kt, dims, es, nd, neighbor, shift, shift, n)
+        assert same(got, want), (case, es, shift)
