@@ -239,25 +239,24 @@ static int32_t tune_group(cycle_t* cy, const std::vector<stage_spec>& specs, con
         for (auto& th : pool) th.join();
     }
     double best_ms = 1e30;
-    hipEvent_t a, b;
-    HIP_TRY(hipEventCreate(&a));
-    HIP_TRY(hipEventCreate(&b));
+    std::vector<std::unique_ptr<length_group>> grps;
+    std::vector<fft_config> kept;
     for (const auto& c : cands) {
         std::unique_ptr<length_group> grp;
         if (make_group(cy, specs, members, c, grp) != DTFFT_HIP_SUCCESS) continue;
-        bool ok = true;
-        for (int i = 0; i < 2 && ok; i++) ok = launch_group(*grp, 0, scratch_in, scratch_out, cy->stream) == 0;
-        if (!ok) continue;
-        hipEventRecord(a, cy->stream);
-        for (int i = 0; i < 5; i++) launch_group(*grp, 0, scratch_in, scratch_out, cy->stream);
-        hipEventRecord(b, cy->stream);
-        hipEventSynchronize(b);
-        float ms = 0;
-        hipEventElapsedTime(&ms, a, b);
-        if (ms < best_ms) { best_ms = ms; best = c; }
+        grps.push_back(std::move(grp));
+        kept.push_back(c);
     }
-    hipEventDestroy(a);
-    hipEventDestroy(b);
+    std::vector<std::function<int32_t()>> runs;
+    for (auto& g : grps) {
+        const length_group* gp = g.get();
+        runs.push_back([gp, scratch_in, scratch_out, cy]() {
+            return launch_group(*gp, 0, scratch_in, scratch_out, cy->stream);
+        });
+    }
+    const std::vector<double> ms = time_candidates(runs, cy->stream);
+    for (size_t i = 0; i < kept.size(); i++)
+        if (ms[i] < best_ms) { best_ms = ms[i]; best = kept[i]; }
     std::string r;
     for (size_t i = 0; i < best.radices.size(); i++) r += (i ? "," : "") + std::to_string(best.radices[i]);
     char buf[160];
@@ -388,7 +387,11 @@ int32_t dtfft_hip_cycle_create(int32_t n_grids, const int32_t* dims, int32_t pre
         const size_t bytes = (size_t)cy->total * (cy->dbl ? 16 : 8);
         HIP_TRY(hipMalloc(&si, bytes));
         if (hipMalloc(&so, bytes) != hipSuccess) { hipFree(si); return DTFFT_HIP_ERROR_ALLOC_FAILED; }
-        if (hipMemsetAsync(si, 0, bytes, cy->stream) != hipSuccess) { hipFree(si); hipFree(so); return DTFFT_HIP_ERROR_HIP_RUNTIME; }
+        if (fill_tuning_scratch(si, bytes, cy->dbl, cy->stream) != DTFFT_HIP_SUCCESS) {
+            hipFree(si);
+            hipFree(so);
+            return DTFFT_HIP_ERROR_HIP_RUNTIME;
+        }
     }
     int32_t rc = DTFFT_HIP_SUCCESS;
     for (int dir = 0; dir < 2 && rc == DTFFT_HIP_SUCCESS; dir++)

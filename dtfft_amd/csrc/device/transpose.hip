@@ -160,3 +160,19 @@ box_transpose_v(void* __restrict__ vout, const void* __restrict__ vin, box_desc 
         __syncthreads();
     }
 }
+
+// Scratch data for the autotuner: finite pseudo-random values in +-[0.5, 1) (FP32 words, or
+// FP64 values as pairs of words), so that candidates are timed on data that toggles as many
+// bits as real input does (an all-zero buffer draws less power, and the GPU's clocks follow it).
+__global__ void __launch_bounds__(256) fill_tuning_data(u32* __restrict__ p, i64 nwords, int dbl) {
+    for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < nwords; i += (i64)gridDim.x * 256) {
+        u32 h = (u32)i * 2654435761u;
+        h ^= h >> 15;
+        h *= 2246822519u;
+        h ^= h >> 13;
+        u32 v;
+        if (dbl) v = (i & 1) ? ((h & 0x800FFFFFu) | 0x3FE00000u) : h;  // high word: sign, exponent of [0.5, 1)
+        else v = (h & 0x807FFFFFu) | 0x3F000000u;
+        p[i] = v;
+    }
+}

@@ -575,22 +575,6 @@ bool parse_radices(const char* text, int n, std::vector<int>& out) {
     return !out.empty() && out.size() <= 10 && prod == n;
 }
 
-static double time_stage(const fft_stage& st, void* in, void* out, hipStream_t s, int warm, int iters) {
-    for (int i = 0; i < warm; i++) if (st.run(in, out, -1, s)) return 1e30;
-    hipEvent_t a, b;
-    if (hipEventCreate(&a) != hipSuccess) return 1e30;
-    if (hipEventCreate(&b) != hipSuccess) { hipEventDestroy(a); return 1e30; }
-    hipEventRecord(a, s);
-    for (int i = 0; i < iters; i++) st.run(in, out, -1, s);
-    hipEventRecord(b, s);
-    hipEventSynchronize(b);
-    float ms = 0;
-    hipEventElapsedTime(&ms, a, b);
-    hipEventDestroy(a);
-    hipEventDestroy(b);
-    return ms / iters;
-}
-
 static int64_t span(const lines_desc& d, bool in, int n) {
     int64_t ls = in ? d.in_ls : d.out_ls, bs = in ? d.in_bs : d.out_bs, ns = in ? d.in_ns : d.out_ns;
     return (d.nl - 1) * ls + (d.nb - 1) * bs + (int64_t)(n - 1) * ns + 1;
@@ -643,7 +627,7 @@ int32_t plan_fft_stage(int n, bool dbl, int64_t nl, int64_t nb, int64_t in_ls, i
         HIP_TRY(hipMalloc(&bi, (size_t)si * es));
         if (!inplace && hipMalloc(&bo, (size_t)so * es) != hipSuccess) { hipFree(bi); return DTFFT_HIP_ERROR_ALLOC_FAILED; }
         if (inplace) bo = bi;
-        HIP_TRY(hipMemsetAsync(bi, 0, (size_t)si * es, stream));
+        TRY(fill_tuning_scratch(bi, (size_t)si * es, dbl, stream));
         double best = 1e30;
         auto cands = fft_candidates(n, dbl, in_cols, out_cols, inplace);
         if (effort < DTFFT_HIP_EXHAUSTIVE && cands.size() > 12) cands.resize(12);
@@ -664,12 +648,19 @@ int32_t plan_fft_stage(int n, bool dbl, int64_t nl, int64_t nb, int64_t in_ls, i
                 if (c.twl != cands[i].twl) cands.push_back(c);
             }
         }
+        std::vector<fft_stage> built;
+        std::vector<fft_config> kept;
         for (auto& c : cands) {
             fft_stage cand;
             if (cand.init(c, d)) continue;
-            double ms = time_stage(cand, bi, bo, stream, 2, 5);
-            if (ms < best) { best = ms; cfg = c; }
+            built.push_back(cand);
+            kept.push_back(c);
         }
+        std::vector<std::function<int32_t()>> runs;
+        for (auto& st : built) runs.push_back([&st, bi, bo, stream]() { return st.run(bi, bo, -1, stream); });
+        const std::vector<double> ms = time_candidates(runs, stream);
+        for (size_t i = 0; i < kept.size(); i++)
+            if (ms[i] < best) { best = ms[i]; cfg = kept[i]; }
         hipStreamSynchronize(stream);
         hipFree(bi);
         if (!inplace) hipFree(bo);

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -47,6 +48,18 @@ const device_info& device();
 bool compile_only();           // no launches / allocations (no device or forced)
 // The configuration set by dtfft_hip_set_config (process-wide, as dtfft_set_config).
 const dtfft_hip_config_t& global_config();
+
+// Times autotuner candidates, each a callable that enqueues one launch on `s` (0 on success):
+// measure_warmup_iters untimed launches each, then rounds in which every candidate in turn
+// runs measure_iters timed launches (src/dtfft_kernel_device.F90:348-349 takes the same two
+// settings).  Returns each candidate's median time per launch over the rounds, in ms (1e30
+// where a launch failed).  Interleaving the candidates keeps the GPU's clock drift under
+// sustained load (durations moving by 20 % over a few launches) out of the comparison.
+std::vector<double> time_candidates(const std::vector<std::function<int32_t()>>& runs, hipStream_t s,
+                                    int rounds = 3);
+// Fills an autotuner scratch buffer with finite pseudo-random values (FP32 words, or FP64
+// values when dbl) on stream s.
+int32_t fill_tuning_scratch(void* p, size_t bytes, bool dbl, hipStream_t s);
 
 // ---- hipRTC program cache ----------------------------------------------------------
 // A compiled kernel: module + function.  `source` is one of the embedded device sources

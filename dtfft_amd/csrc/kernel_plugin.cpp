@@ -107,23 +107,6 @@ static void default_shape(int es, bool transpose, int& tile, int& rows, int& pad
     }
 }
 
-static double time_mover(const mover& mv, const box_desc& d, void* in, void* out, hipStream_t s,
-                         int warm, int iters) {
-    for (int i = 0; i < warm; i++) if (mv.run(d, in, out, s)) return 1e30;
-    hipEvent_t a, b;
-    if (hipEventCreate(&a) != hipSuccess) return 1e30;
-    if (hipEventCreate(&b) != hipSuccess) { hipEventDestroy(a); return 1e30; }
-    hipEventRecord(a, s);
-    for (int i = 0; i < iters; i++) mv.run(d, in, out, s);
-    hipEventRecord(b, s);
-    hipEventSynchronize(b);
-    float ms = 0;
-    hipEventElapsedTime(&ms, a, b);
-    hipEventDestroy(a);
-    hipEventDestroy(b);
-    return ms / iters;
-}
-
 int32_t tune_mover(const box_move& m, int effort, bool force_effort, hipStream_t stream, mover* out) {
     int tile, rows, pad, vec;
     default_shape(m.es, m.transpose, tile, rows, pad, vec);
@@ -148,6 +131,11 @@ int32_t tune_mover(const box_move& m, int effort, bool force_effort, hipStream_t
         void *bi = nullptr, *bo = nullptr;
         HIP_TRY(hipMalloc(&bi, (size_t)span_in * m.es));
         if (hipMalloc(&bo, (size_t)span_out * m.es) != hipSuccess) { hipFree(bi); return DTFFT_HIP_ERROR_ALLOC_FAILED; }
+        if (fill_tuning_scratch(bi, (size_t)span_in * m.es, false, stream) != DTFFT_HIP_SUCCESS) {
+            hipFree(bi);
+            hipFree(bo);
+            return DTFFT_HIP_ERROR_HIP_RUNTIME;
+        }
         double best = 1e30;
         const int tiles[] = {16, 32, 64};
         const int rowss[] = {1, 2, 4, 8, 16, 32};
@@ -158,6 +146,8 @@ int32_t tune_mover(const box_move& m, int effort, bool force_effort, hipStream_t
         const bool vec_ok = V > 1 && aligned_for_transpose(d, V, bi, bo, m.es);
         std::vector<int> vecs = {1};
         if (vec_ok) vecs.push_back(V);
+        std::vector<mover> cands;
+        std::vector<std::array<int, 4>> shapes;
         for (int v : vecs)
             for (int t : tiles)
                 for (int r : rowss)
@@ -168,9 +158,17 @@ int32_t tune_mover(const box_move& m, int effort, bool force_effort, hipStream_t
                         if (effort < DTFFT_HIP_EXHAUSTIVE && p == 2) continue;
                         mover cand;
                         if (cand.init(m.es, true, t, r, p, v)) continue;
-                        double ms = time_mover(cand, d, bi, bo, stream, 2, 5);
-                        if (ms < best) { best = ms; tile = t; rows = r; pad = p; tvec = v; }
+                        cands.push_back(cand);
+                        shapes.push_back({t, r, p, v});
                     }
+        std::vector<std::function<int32_t()>> runs;
+        for (auto& mv : cands) runs.push_back([&mv, &d, bi, bo, stream]() { return mv.run(d, bi, bo, stream); });
+        const std::vector<double> ms = time_candidates(runs, stream);
+        for (size_t i = 0; i < cands.size(); i++)
+            if (ms[i] < best) {
+                best = ms[i];
+                tile = shapes[i][0]; rows = shapes[i][1]; pad = shapes[i][2]; tvec = shapes[i][3];
+            }
         hipStreamSynchronize(stream);
         hipFree(bi);
         hipFree(bo);

@@ -17,6 +17,8 @@
 #include <fstream>
 #include <sstream>
 
+#include <algorithm>
+
 #include "internal.h"
 
 #include "device_sources.inc"  // generated at build: k_src_common, k_src_fft, k_src_transpose
@@ -229,6 +231,51 @@ int32_t launch(const rtc_kernel* k, unsigned grid, unsigned block, hipStream_t s
     if (grid == 0) return DTFFT_HIP_SUCCESS;
     HIP_TRY(hipModuleLaunchKernel(k->fn, grid, 1, 1, block, 1, 1, 0, stream, args, nullptr));
     return DTFFT_HIP_SUCCESS;
+}
+
+// ---- candidate timing -----------------------------------------------------------
+std::vector<double> time_candidates(const std::vector<std::function<int32_t()>>& runs, hipStream_t s, int rounds) {
+    const dtfft_hip_config_t& c = global_config();
+    const int warm = std::max(0, c.measure_warmup_iters), iters = std::max(1, c.measure_iters);
+    const size_t n = runs.size();
+    std::vector<double> out(n, 1e30);
+    std::vector<std::vector<float>> t(n);
+    std::vector<char> ok(n, 1);
+    for (size_t i = 0; i < n; i++)
+        for (int w = 0; w < warm && ok[i]; w++) ok[i] = runs[i]() == 0;
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess) return out;
+    if (hipEventCreate(&b) != hipSuccess) { hipEventDestroy(a); return out; }
+    for (int r = 0; r < rounds; r++)
+        for (size_t i = 0; i < n; i++) {
+            if (!ok[i]) continue;
+            hipEventRecord(a, s);
+            for (int k = 0; k < iters && ok[i]; k++) ok[i] = runs[i]() == 0;
+            hipEventRecord(b, s);
+            hipEventSynchronize(b);
+            float ms = 0;
+            hipEventElapsedTime(&ms, a, b);
+            if (ok[i]) t[i].push_back(ms / iters);
+        }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    for (size_t i = 0; i < n; i++) {
+        if (!ok[i] || t[i].empty()) continue;
+        std::vector<float> v = t[i];
+        std::sort(v.begin(), v.end());
+        out[i] = v[v.size() / 2];
+    }
+    return out;
+}
+
+int32_t fill_tuning_scratch(void* p, size_t bytes, bool dbl, hipStream_t s) {
+    const rtc_kernel* k = nullptr;
+    TRY(rtc_get("transpose", "fill_tuning_data", &k));
+    int64_t nwords = (int64_t)(bytes / 4);
+    int d = dbl ? 1 : 0;
+    if (nwords == 0) return DTFFT_HIP_SUCCESS;
+    void* args[] = {&p, &nwords, &d};
+    return launch(k, (unsigned)std::min<int64_t>((nwords + 255) / 256, 1 << 16), 256, s, args);
 }
 
 // ---- tuning records ------------------------------------------------------------
