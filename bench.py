@@ -326,27 +326,41 @@ def config1(D, args, timer):
 
 
 def config6(D, args, timer):
-    """3D R2R 512^3 FP32: DCT-2 along every axis forward, DCT-3 (its inverse) backward; the
-    reference's plain sequence with R2R stages (one fused kernel each: pre-process, C2C and
-    post-process through LDS) and permutes between them."""
+    """3D R2R 512^3 FP32: DCT-2 along every axis forward, DCT-3 (its inverse) backward.  The
+    fused sequence folds each transpose into the R2R stage that follows it (3 launches per
+    direction); the plain one is the reference's sequence, R2R stages with permutes between
+    them (5 launches per direction).  Every R2R stage is one kernel: pre-process, half-length
+    C2C and post-process through LDS."""
     n = 512
     N = n ** 3
-    plan = D.PlanR2R([n, n, n], [5, 5, 5], D.Precision.SINGLE, D.Effort(args.effort), D.Executor.HIP)
+    plans = {}
+    for fused in (True, False):
+        D.set_config(enable_fused=fused)
+        try:
+            plans[fused] = D.PlanR2R([n, n, n], [5, 5, 5], D.Precision.SINGLE, D.Effort(args.effort),
+                                     D.Executor.HIP)
+        finally:
+            D.set_config()
     a, b, c, aux = (D.DeviceArray(N * 4) for _ in range(4))
     a.upload(np.random.default_rng(6).standard_normal(N).astype(np.float32))
 
-    def cyc():
+    def cyc(plan):
         plan.execute(a, b, D.Execute.FORWARD, aux)
         plan.execute(b, c, D.Execute.BACKWARD, aux)
-    ms = _time(D, timer, cyc, args.steps, args.warmup)
+    ms = _time(D, timer, lambda: cyc(plans[True]), args.steps, args.warmup)
+    ms_p = _time(D, timer, lambda: cyc(plans[False]), max(3, args.steps // 2), 2)
     algo = 12 * N * 4
     err = None
     if not args.no_cpu:
-        plan.execute(a, b, D.Execute.FORWARD, aux)
+        plans[True].execute(a, b, D.Execute.FORWARD, aux)
         err = check_r2r(a.download((n, n, n), np.float32), b.download((n, n, n), np.float32))
     _line("R2R fwd+bwd cycle GB/s, 512^3 fp32 DCT-2/DCT-3", algo / (ms * 1e-3) / 1e9, "GB/s", ms, args,
-          "3D R2R 512^3 float32, DCT-2 on X, Y, Z forward, DCT-3 backward, with permutes",
-          dtype="f32", algorithmic_bytes=algo, forward_rel_l2=err,
+          "3D R2R 512^3 float32, DCT-2 on X, Y, Z forward, DCT-3 backward, transposes fused into the "
+          "R2R stages that follow them",
+          dtype="f32", algorithmic_bytes=algo,
+          plain={"value": round(algo / (ms_p * 1e-3) / 1e9, 2), "ms_per_step": round(ms_p, 4),
+                 "form": "the reference's sequence: R2R stages with permutes between them"},
+          fused_over_plain=round(ms_p / ms, 3), forward_rel_l2=err,
           forward_rel_l2_against="FP64 oracle (FFTW's REDFT10 as a dense matrix) along X, Y, Z")
 
 

@@ -150,7 +150,8 @@ struct r2r_len {
 // Z_k = (V_k + conj V_{M-k}) + i (V_k - conj V_{M-k}) conj(W^k), k < M, whose backward
 // M-point transform is v_2j + i v_2j+1 (the C2R folding of fft_c2r_kernel).  rtwtab holds
 // W^k, k <= M (gen_rtwiddles); it is unused otherwise.
-template <typename T, class S, int TPL, int LPB, int PAD, int KIND, bool FULL, bool HALF = false>
+template <typename T, class S, int TPL, int LPB, int PAD, int KIND, bool FULL, bool HALF = false,
+          bool IN_COLS = false, bool OUT_COLS = false>
 __global__ void __launch_bounds__(TPL * LPB)
 fft_r2r_kernel(const T* __restrict__ in, T* __restrict__ out, const typename tw_of<T>::type* __restrict__ tw,
                const cx<T>* __restrict__ t4tab, const typename tw_of<T>::type* __restrict__ rtwtab, r2r_desc d) {
@@ -160,14 +161,19 @@ fft_r2r_kernel(const T* __restrict__ in, T* __restrict__ out, const typename tw_
     constexpr int NX = HALF ? 2 * M : r2r_len<KIND, M>::n;
     constexpr bool INV = (KIND == 4 || KIND == 8);  // types 3 run the backward transform
     static_assert(!HALF || KIND == 4 || KIND == 5 || KIND == 8 || KIND == 9, "half length: types 2 and 3");
+    static_assert(!(IN_COLS || OUT_COLS) || FULL, "column layouts run whole tiles");
     constexpr int XIT = (LPB * NX + F::NT - 1) / F::NT;  // real elements per thread
     constexpr int MIT = (LPB * M + F::NT - 1) / F::NT;   // work elements per thread
     constexpr int TWN = S::TW_SIZE > 0 ? S::TW_SIZE : 1;
+    // The column forms run 1024-thread blocks at the 128-VGPR limit: their pre- and
+    // post-process loops stay rolled, which keeps the FP32 form's spills to a few registers
+    // (61-72 fully unrolled).
+    constexpr int MU = (IN_COLS || OUT_COLS) ? 1 : MIT, XU = (IN_COLS || OUT_COLS) ? 1 : XIT;
     __shared__ cx<T> lds[F::LDS_ELEMS];
-    __shared__ T xs[LPB * NX];
+    __shared__ T xs[LPB * NX];  // real lines: [line][j], or [j][line] when read across lines
     __shared__ tw_t twl[TWN];
     __shared__ cx<T> t4l[2 * NX];
-    const i64 lines = d.nl, ntiles = (lines + LPB - 1) / LPB;
+    const i64 lines = d.nl * d.nb, ntiles = (lines + LPB - 1) / LPB;
     dtfft::lines_desc ld{};  // only nl is read by the LDS-to-LDS passes
     ld.nl = lines;
     for (int i = threadIdx.x; i < S::TW_SIZE; i += F::NT) twl[i] = tw[i];
@@ -176,16 +182,33 @@ fft_r2r_kernel(const T* __restrict__ in, T* __restrict__ out, const typename tw_
     __shared__ tw_t rtwl[HALF ? M + 1 : 1];
     if constexpr (HALF)
         for (int i = threadIdx.x; i <= M; i += F::NT) rtwl[i] = rtwtab[i];
-    // element q of this thread in tile t: line (tid + q NT) / NX, position (tid + q NT) % NX
+    // Line li of the stage is line li % nl of batch li / nl.  Rows: element j of a line at
+    // j, lines in_ls apart (the host folds evenly spaced batches into one range).  Columns
+    // (IN_COLS / OUT_COLS): LPB adjacent lines (ls 1), element j at j * es; nl is a multiple
+    // of LPB, so a tile never straddles two batches.
+    auto in_base = [&](i64 l0) { const i64 b = l0 / d.nl; return b * d.in_bs + (l0 - b * d.nl) * d.in_ls; };
+    auto out_base = [&](i64 l0) { const i64 b = l0 / d.nl; return b * d.out_bs + (l0 - b * d.nl) * d.out_ls; };
+    // element q of this thread in a tile: rows (line, j) = ((tid + q NT) / NX, % NX); columns
+    // (line, j) = ((tid + q NT) % LPB, / LPB), so a wave reads whole LPB-line pieces of rows
     T pf[XIT];
     auto fetch = [&](i64 t) {
         const i64 l0 = (t < ntiles ? t : ntiles - 1) * LPB;
+        if constexpr (IN_COLS) {
+            const T* src = in + in_base(l0);
 #pragma unroll
-        for (int q = 0; q < XIT; q++) {
-            const int e = (int)threadIdx.x + q * F::NT;
-            const int line = e / NX, j = e % NX;
-            const i64 li = l0 + line < lines ? l0 + line : lines - 1;
-            pf[q] = in[li * d.in_ls + (XIT * F::NT == LPB * NX || e < LPB * NX ? j : 0)];
+            for (int q = 0; q < XIT; q++) {
+                const int e = (int)threadIdx.x + q * F::NT;
+                const int ee = XIT * F::NT == LPB * NX || e < LPB * NX ? e : 0;
+                pf[q] = src[(i64)(ee % LPB) * d.in_ls + (i64)(ee / LPB) * d.in_es];
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < XIT; q++) {
+                const int e = (int)threadIdx.x + q * F::NT;
+                const int line = e / NX, j = e % NX;
+                const i64 li = l0 + line < lines ? l0 + line : lines - 1;
+                pf[q] = in[li * d.in_ls + (XIT * F::NT == LPB * NX || e < LPB * NX ? j : 0)];
+            }
         }
     };
     auto stage_lines = [&]() {  // the prefetched lines into LDS
@@ -202,13 +225,13 @@ fft_r2r_kernel(const T* __restrict__ in, T* __restrict__ out, const typename tw_
         const i64 l0 = t * LPB;
         fetch(t + gridDim.x);
         __syncthreads();  // the tile's lines (and, the first time, the twiddles) are in LDS
-#pragma unroll
+#pragma unroll MU
         for (int q = 0; q < MIT; q++) {
             const int e = (int)threadIdx.x + q * F::NT;
             if (MIT * F::NT == LPB * M || e < LPB * M) {
-                const int line = e / M, m = e % M;
-                const T* x = xs + line * NX;
-                auto xf = [&](i64 j) { return x[j]; };
+                // columns: lines fastest, so the reads of a wave go across xs's line axis
+                const int line = IN_COLS ? e % LPB : e / M, m = IN_COLS ? e / LPB : e % M;
+                auto xf = [&](i64 j) { return IN_COLS ? xs[j * LPB + line] : xs[line * NX + j]; };
                 cx<T> z;
                 if constexpr (!HALF) {
                     z = r2r_pre_elem<T, KIND>(xf, (i64)NX, (i64)M, (i64)m, t4);
@@ -232,11 +255,13 @@ fft_r2r_kernel(const T* __restrict__ in, T* __restrict__ out, const typename tw_
         __syncthreads();
         F::template passes<0, false, false>(nullptr, nullptr, twl, lds, 0, 0, ld, l0, false);
         __syncthreads();
-#pragma unroll
+        const i64 ob = (IN_COLS || OUT_COLS) ? out_base(l0) : 0;
+#pragma unroll XU
         for (int q = 0; q < XIT; q++) {
             const int e = (int)threadIdx.x + q * F::NT;
             if (XIT * F::NT == LPB * NX || e < LPB * NX) {
-                const int line = e / NX, k = e % NX;
+                // rows out: a wave writes consecutive k of a line; columns out: lines fastest
+                const int line = OUT_COLS ? e % LPB : e / NX, k = OUT_COLS ? e / LPB : e % NX;
                 const i64 li = l0 + line;
                 if (FULL || li < lines) {
                     const cx<T>* W = lds + line * F::LD;
@@ -268,7 +293,8 @@ fft_r2r_kernel(const T* __restrict__ in, T* __restrict__ out, const typename tw_
                         };
                         y = r2r_post_elem<T, KIND>(vf, (i64)NX, (i64)k, t4);
                     }
-                    out[li * d.out_ls + k] = y;
+                    if constexpr (IN_COLS || OUT_COLS) out[ob + (i64)line * d.out_ls + (i64)k * d.out_es] = y;
+                    else out[li * d.out_ls + k] = y;
                 }
             }
         }

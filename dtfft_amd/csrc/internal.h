@@ -197,6 +197,13 @@ struct r2r_stage {
     unsigned fgrid = 0, fblock = 0;
     int32_t init(int kind, int64_t n, bool dbl, int64_t nl, int64_t nb, int64_t in_ls, int64_t in_bs,
                  int64_t in_es, int64_t out_ls, int64_t out_bs, int64_t out_es, int effort, hipStream_t stream);
+    // The fused kernel reading its lines across (nl adjacent lines per batch, element stride
+    // in_es) and writing rows (out_es 1) or, with out_cols, columns (out_ls 1) -- the transpose
+    // before the stage fused into its load.  Types 2 / 3 of even length through the half-length
+    // form; *ok is false (nothing set up) where the stage does not have that form or nl is not a
+    // multiple of the 128-byte tile row.
+    int32_t init_cols(int kind, int64_t n, bool dbl, int64_t nl, int64_t nb, int64_t in_bs, int64_t in_es,
+                      int64_t out_ls, int64_t out_bs, int64_t out_es, bool out_cols, bool* ok);
     size_t work_bytes() const;
     int32_t run(const void* in, void* out, void* work, hipStream_t s) const;
 };

@@ -70,6 +70,9 @@ struct plan_t {
     bool r2r = false;
     int kind[3] = {0, 0, 0};                         // R2R kinds along X, Y, Z
     r2r_stage rx, ry, rz, rxb, ryb, rzb;             // R2R forward / backward stages
+    // fused R2R sequence: the transposes folded into the stages reading across lines
+    r2r_stage ryc, rzc, rxbc, rybc;
+    bool r2r_fused = false;
     void* work = nullptr;                            // complex work lines of the R2R stages
     // forward + backward captured as one hipGraph for one set of pointers
     hipStream_t gstream = nullptr;
@@ -181,6 +184,27 @@ static int32_t build_r2r(plan_t* p) {
         wb = std::max({wb, p->rz.work_bytes(), p->rzb.work_bytes()});
     }
     if (!compile_only()) HIP_TRY(hipMalloc(&p->work, std::max<size_t>(wb, 16)));
+    // Fused sequence, as the fused C2C one: forward X rows, X->Y + Y, Y->Z + Z; backward Z
+    // rows, Z->X + X, Y in place on the X-pencil (the axes of a separable R2R commute).  Taken
+    // when every stage that reads across lines has the fused form (types 2 / 3 of even length,
+    // lines in whole 128-byte tile rows); otherwise the reference's sequence with permutes.
+    if (p->fused) {
+        const int k0 = p->kind[0], k1 = p->kind[1];
+        const int i0 = r2r_inverse_kind(k0), i1 = r2r_inverse_kind(k1);
+        bool a = false, b = false, c = false, e = false;
+        if (p->ndims == 3) {
+            const int k2 = p->kind[2];
+            TRY(p->ryc.init_cols(k1, ny, p->dbl, nx, nz, nx * ny, nx, ny * nz, ny, 1, false, &a));
+            TRY(p->rzc.init_cols(k2, nz, p->dbl, ny, nx, ny * nz, ny, nz * nx, nz, 1, false, &b));
+            TRY(p->rxbc.init_cols(i0, nx, p->dbl, nz, ny, nz * nx, nz, nx * ny, nx, 1, false, &c));
+            TRY(p->rybc.init_cols(i1, ny, p->dbl, nx, nz, nx * ny, nx, 1, nx * ny, nx, true, &e));
+            p->r2r_fused = a && b && c && e;
+        } else {
+            TRY(p->ryc.init_cols(k1, ny, p->dbl, nx, 1, 0, nx, ny, 0, 1, false, &a));
+            TRY(p->rxbc.init_cols(i0, nx, p->dbl, ny, 1, 0, ny, nx, 0, 1, false, &c));
+            p->r2r_fused = a && c;
+        }
+    }
     return DTFFT_HIP_SUCCESS;
 }
 
@@ -291,11 +315,11 @@ static int32_t create_plan(int8_t ndims, const int32_t* dims, int32_t precision,
     p->elem_size = r2c ? p->cplx / 2 : p->cplx;
     if (r2r) {
         // real elements throughout; one rank, so the generic order (Z-slab gives the same
-        // Z-pencil) and no fusion
+        // Z-pencil); fused transposes where every stage has the fused form (build_r2r)
         p->r2r = !p->transpose_only;
         p->cplx = p->elem_size = p->dbl ? 8 : 4;
         p->z_slab = false;
-        p->fused = false;
+        p->fused = p->r2r && g_config.enable_fused != 0;
         if (p->r2r) {
             if (!kinds) return DTFFT_HIP_ERROR_MISSING_R2R_KINDS;
             for (int i = 0; i < ndims; i++) {
@@ -393,6 +417,26 @@ static int32_t exec_r2r(plan_t* p, void* in, void* out, bool fwd, void* aux, ste
     hipStream_t s = p->stream;
     void* w = p->work;
     TRY(ensure_aux(p, aux));
+    if (p->r2r_fused) {
+        if (p->ndims == 2) {
+            if (fwd) {
+                STEP(r, p->rx.run(in, aux, w, s));
+                STEP(r, p->ryc.run(aux, out, w, s));
+            } else {
+                STEP(r, p->ryb.run(in, aux, w, s));
+                STEP(r, p->rxbc.run(aux, out, w, s));
+            }
+        } else if (fwd) {
+            STEP(r, p->rx.run(in, out, w, s));
+            STEP(r, p->ryc.run(out, aux, w, s));
+            STEP(r, p->rzc.run(aux, out, w, s));
+        } else {
+            STEP(r, p->rzb.run(in, aux, w, s));
+            STEP(r, p->rxbc.run(aux, out, w, s));
+            STEP(r, p->rybc.run(out, out, w, s));
+        }
+        return DTFFT_HIP_SUCCESS;
+    }
     if (p->ndims == 2) {
         if (fwd) {
             STEP(r, p->rx.run(in, aux, w, s));
@@ -698,9 +742,18 @@ int32_t dtfft_hip_report(dtfft_hip_plan_t plan) {
     auto* p = (plan_t*)plan;
     if (!p) return DTFFT_HIP_ERROR_PLAN_NOT_CREATED;
     std::printf("dtfft_hip plan: %dD %s %s, dims (X fastest) %lld x %lld x %lld, %s%s%s\n", p->ndims,
-                p->r2c ? "R2C" : "C2C", p->dbl ? "double" : "single", (long long)p->nx, (long long)p->ny,
-                (long long)p->nz, p->transpose_only ? "transpose-only" : "HIP executor",
-                p->z_slab ? ", Z-slab" : "", p->fused ? ", fused transposes" : "");
+                p->r2r ? "R2R" : p->r2c ? "R2C" : "C2C", p->dbl ? "double" : "single", (long long)p->nx,
+                (long long)p->ny, (long long)p->nz, p->transpose_only ? "transpose-only" : "HIP executor",
+                p->z_slab ? ", Z-slab" : "",
+                (p->r2r ? p->r2r_fused : p->fused) ? ", fused transposes" : "");
+    auto show_r2r = [](const char* name, const r2r_stage& st) {
+        if (st.fused && st.fk) std::printf("  %-8s %s\n", name, st.fk->expr.c_str());
+    };
+    show_r2r("r2r_x", p->rx); show_r2r("r2r_zb", p->rzb); show_r2r("r2r_yb", p->ryb);
+    if (p->r2r_fused) {
+        show_r2r("xy+r2r", p->ryc); show_r2r("yz+r2r", p->rzc); show_r2r("zx+r2r", p->rxbc);
+        show_r2r("r2r_ycol", p->rybc);
+    }
     auto show = [](const char* name, const fft_stage& st) {
         if (!st.k) return;
         std::printf("  %-8s %s\n", name, st.cfg.expr().c_str());

@@ -137,6 +137,55 @@ int32_t r2r_stage::init(int kind_, int64_t n_, bool dbl_, int64_t nl, int64_t nb
 
 size_t r2r_stage::work_bytes() const { return fused ? 0 : (size_t)(d.nl * d.nb * d.wdist) * (dbl ? 16 : 8); }
 
+int32_t r2r_stage::init_cols(int kind_, int64_t n_, bool dbl_, int64_t nl, int64_t nb, int64_t in_bs, int64_t in_es,
+                             int64_t out_ls, int64_t out_bs, int64_t out_es, bool out_cols, bool* ok) {
+    *ok = false;
+    if (!r2r_kind_valid(kind_) || n_ < 8 || n_ % 2 != 0) return DTFFT_HIP_SUCCESS;
+    if (kind_ != 4 && kind_ != 5 && kind_ != 8 && kind_ != 9) return DTFFT_HIP_SUCCESS;
+    std::vector<int> hrad;
+    if (!half_schedule((int)(n_ / 2), hrad)) return DTFFT_HIP_SUCCESS;
+    const size_t ec = dbl_ ? 16 : 8, er = dbl_ ? 8 : 4;
+    const int lpb = (int)(128 / er);  // a tile row is one 128-byte piece of LPB reals
+    if (nl % lpb != 0) return DTFFT_HIP_SUCCESS;
+    const int64_t mf = n_ / 2;
+    fft_config c = default_fft_config((int)mf, dbl_, false, false, false);
+    c.radices = hrad;
+    c.pad = 1;
+    c.lpb = lpb;
+    const int rmax = *std::max_element(hrad.begin(), hrad.end());
+    c.tpl = (int)std::max<int64_t>(1, std::min<int64_t>(mf / std::max(4, rmax), 1024 / lpb));
+    const size_t lds = (size_t)lpb * (size_t)((mf + c.pad)) * ec + (size_t)lpb * n_ * er +
+                       (size_t)std::max<int64_t>(1, mf - 1) * 16 + (size_t)(2 * n_) * ec + (size_t)(mf + 1) * 16;
+    if (lds > 160 * 1024) return DTFFT_HIP_SUCCESS;
+    kind = kind_;
+    n = n_;
+    dbl = dbl_;
+    m = work_length(kind, n);
+    sign = (kind == 4 || kind == 8) ? DTFFT_HIP_FFT_BACKWARD : DTFFT_HIP_FFT_FORWARD;
+    d = r2r_desc_h{nl, nb, 1, in_bs, in_es, out_ls, out_bs, out_es, n, m, m};
+    std::string e = std::string("fft_r2r_kernel<") + (dbl ? "double" : "float") + ", dtfft::sched<" +
+                    std::to_string(mf);
+    for (int r : c.radices) e += ", " + std::to_string(r);
+    e += ">, " + std::to_string(c.tpl) + ", " + std::to_string(c.lpb) + ", " + std::to_string(c.pad) + ", " +
+         std::to_string(kind) + ", true, true, true" + (out_cols ? ", true>" : ", false>");
+    TRY(rtc_get("r2r", e, &fk));
+    TRY(get_twiddles((int)mf, c.radices, dbl, &ftw));
+    TRY(get_r2r_table(n, dbl, &t4));
+    TRY(get_rtwiddles((int)n, dbl, &rtw));
+    half = true;
+    const int64_t tiles = nl * nb / lpb;
+    fblock = (unsigned)(c.tpl * c.lpb);
+    int per_cu = 1;
+    if (!compile_only() && fk && fk->fn &&
+        hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fk->fn, (int)fblock, 0) != hipSuccess)
+        per_cu = 1;
+    per_cu = std::max(1, per_cu);
+    fgrid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)per_cu * std::max(1, device().num_cus)));
+    fused = true;
+    *ok = true;
+    return DTFFT_HIP_SUCCESS;
+}
+
 // e^{-i pi j / 4n} for j < 2n, generated on the device in FP64; one table per (n, precision)
 // for the life of the process.
 static std::mutex g_t4_mu;

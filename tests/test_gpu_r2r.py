@@ -171,3 +171,79 @@ def test_executor_rank2_padded_batches(prec):
     want = R.r2r(g, kinds[1])
     want = np.swapaxes(R.r2r(np.swapaxes(want, 1, 2), kinds[0]), 1, 2)
     assert rel(y, want) < TOL[prec]
+
+
+def _plan_r2r(dims, kinds, prec, fused, capfd):
+    D.set_config(enable_fused=fused)
+    try:
+        p = D.PlanR2R(dims, kinds, prec, D.Effort.ESTIMATE, D.Executor.HIP)
+    finally:
+        D.set_config()
+    capfd.readouterr()
+    p.report()
+    return p, "fused transposes" in capfd.readouterr().out
+
+
+@pytest.mark.parametrize("prec", [P.SINGLE, P.DOUBLE])
+@pytest.mark.parametrize("dims,kinds", [((64, 32, 96), (5, 5, 5)), ((32, 64, 32), (4, 9, 8)),
+                                        ((96, 32, 64), (8, 4, 5))])
+def test_plan_r2r_3d_fused_transposes(dims, kinds, prec, capfd):
+    """The fused R2R sequence (transposes folded into the stages that read across lines: X->Y
+    + R2R_Y, Y->Z + R2R_Z forward; Z->X + R2R_X and R2R_Y in place backward) against the
+    oracle's reference sequence, forward and backward, and against the plan's own sequence with
+    permutes (the same butterflies; a few units in the last place apart)."""
+    nx, ny, nz = dims
+    rng = np.random.default_rng(nx * 7 + ny * 3 + nz)
+    x = rng.standard_normal((nz, ny, nx)).astype(rdt(prec))
+    pf, is_fused = _plan_r2r(dims, kinds, prec, True, capfd)
+    pp, plain_fused = _plan_r2r(dims, kinds, prec, False, capfd)
+    assert is_fused and not plain_fused
+    outs = {}
+    for name, p in (("fused", pf), ("plain", pp)):
+        a = D.DeviceArray.from_numpy(x)
+        b = D.DeviceArray(x.nbytes)
+        p.execute(a, b, D.Execute.FORWARD)
+        outs[name] = b.download((ny, nx, nz), x.dtype).astype(np.float64)
+    assert rel(outs["fused"], R.execute_generic_r2r(x.astype(np.float64), kinds, True)) < TOL[prec]
+    assert rel(outs["fused"], outs["plain"]) < TOL[prec] / 4
+    y = rng.standard_normal((ny, nx, nz)).astype(rdt(prec))
+    for name, p in (("fused", pf), ("plain", pp)):
+        a = D.DeviceArray.from_numpy(y)
+        b = D.DeviceArray(y.nbytes)
+        p.execute(a, b, D.Execute.BACKWARD)
+        outs[name] = b.download((nz, ny, nx), y.dtype).astype(np.float64)
+    assert rel(outs["fused"], R.execute_generic_r2r(y.astype(np.float64), kinds, False)) < TOL[prec]
+    assert rel(outs["fused"], outs["plain"]) < TOL[prec] / 4
+
+
+@pytest.mark.parametrize("prec", [P.SINGLE, P.DOUBLE])
+@pytest.mark.parametrize("dims,kinds", [((64, 96), (5, 4)), ((32, 32), (9, 8))])
+def test_plan_r2r_2d_fused_transposes(dims, kinds, prec, capfd):
+    nx, ny = dims
+    rng = np.random.default_rng(nx + ny)
+    x = rng.standard_normal((ny, nx)).astype(rdt(prec))
+    pf, is_fused = _plan_r2r(dims, kinds, prec, True, capfd)
+    assert is_fused
+    a = D.DeviceArray.from_numpy(x)
+    b = D.DeviceArray(x.nbytes)
+    pf.execute(a, b, D.Execute.FORWARD)
+    assert rel(b.download((nx, ny), x.dtype).astype(np.float64), R.execute_2d_r2r(x.astype(np.float64), kinds, True)) < TOL[prec]
+    y = rng.standard_normal((nx, ny)).astype(rdt(prec))
+    a = D.DeviceArray.from_numpy(y)
+    pf.execute(a, b, D.Execute.BACKWARD)
+    assert rel(b.download((ny, nx), y.dtype).astype(np.float64), R.execute_2d_r2r(y.astype(np.float64), kinds, False)) < TOL[prec]
+
+
+@pytest.mark.parametrize("dims,kinds", [((48, 32, 64), (5, 5, 5)), ((64, 32, 64), (5, 6, 5))])
+def test_plan_r2r_fused_falls_back_to_permutes(dims, kinds, capfd):
+    """A stage without the fused column form (a line count that is not a multiple of the
+    32-line tile row, a DCT-4) keeps the sequence with permutes; it still matches the oracle."""
+    nx, ny, nz = dims
+    x = np.random.default_rng(9).standard_normal((nz, ny, nx)).astype(np.float32)
+    p, is_fused = _plan_r2r(dims, kinds, P.SINGLE, True, capfd)
+    assert not is_fused
+    a = D.DeviceArray.from_numpy(x)
+    b = D.DeviceArray(x.nbytes)
+    p.execute(a, b, D.Execute.FORWARD)
+    z = b.download((ny, nx, nz), x.dtype).astype(np.float64)
+    assert rel(z, R.execute_generic_r2r(x.astype(np.float64), kinds, True)) < TOL[P.SINGLE]
