@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--config", type=int, action="append", default=[],
                     help="also run another workload: 1 2D c128, 2 transpose c128, 3 R2C, 4 batched, 5 pencil share, "
-                         "6 R2R")
+                         "6 R2R, 7 the metric's FP64 twin")
     ap.add_argument("--only-configs", action="store_true", help="skip the headline metric")
     return ap.parse_args()
 
@@ -120,7 +120,7 @@ def main():
         timer = Timer(D)
         for cfg in args.config:
             log(f"[bench] config {cfg}")
-            {1: config1, 2: config2, 3: config3, 4: config4, 5: config5, 6: config6}[cfg](D, args, timer)
+            {1: config1, 2: config2, 3: config3, 4: config4, 5: config5, 6: config6, 7: config7}[cfg](D, args, timer)
         if args.only_configs:
             return
 
@@ -323,6 +323,43 @@ def config1(D, args, timer):
               algo / (ms * 1e-3) / 1e9, "GB/s", ms, args,
               "2D C2C 256x256 complex128, one rank, fwd+bwd", dtype="c128", algorithmic_bytes=algo,
               roundtrip_rel_l2=err, cycles_per_s=round(1000.0 / ms, 1))
+
+
+def config7(D, args, timer):
+    """3D C2C 512^3 complex128 (the FP64 twin of the metric): the fused cycle against the
+    plain (reference) sequence, fwd+bwd, with the round trip bwd(fwd(x)) / N against x."""
+    n = 512
+    N = n ** 3
+    rng = np.random.default_rng(7)
+    x = np.empty(N, dtype=np.complex128)
+    x.real = rng.uniform(-1, 1, N)
+    x.imag = rng.uniform(-1, 1, N)
+    a = D.DeviceArray.from_numpy(x)
+    b, c, aux = D.DeviceArray(N * 16), D.DeviceArray(N * 16), D.DeviceArray(N * 16)
+    algo = 12 * N * 16
+    res = {}
+    for fused in (True, False):
+        D.set_config(enable_fused=fused)
+        try:
+            plan = D.PlanC2C([n, n, n], D.Precision.DOUBLE, D.Effort(args.effort), D.Executor.HIP)
+        finally:
+            D.set_config()
+
+        def cyc():
+            plan.execute(a, b, D.Execute.FORWARD, aux)
+            plan.execute(b, c, D.Execute.BACKWARD, aux)
+        res[fused] = _time(D, timer, cyc, max(3, args.steps // 2), 2)
+        if fused:
+            back = c.download(N, np.complex128)
+            rt = _rel(back / N, x)
+            del back
+        del plan
+    _line("fwd+bwd cycle GB/s, 512^3 C2C fp64", algo / (res[True] * 1e-3) / 1e9, "GB/s", res[True], args,
+          "3D C2C 512x512x512 complex128, one forward + one backward pass, transposes fused into the "
+          "following FFT pass", dtype="c128", algorithmic_bytes=algo,
+          plain={"value": round(algo / (res[False] * 1e-3) / 1e9, 2), "ms_per_step": round(res[False], 4),
+                 "form": "the reference's sequence, separate FFT and permute launches"},
+          fused_over_plain=round(res[False] / res[True], 3), roundtrip_rel_l2=rt)
 
 
 def config6(D, args, timer):
