@@ -685,7 +685,24 @@ int32_t plan_fft_stage(int n, bool dbl, int64_t nl, int64_t nb, int64_t in_ls, i
         c.dma = lds_dma == 2 ? 1 : lds_dma == 3 ? 2 : (c.out_cols ? 1 : 2);
         c.lpb = 128 / elem_bytes(dbl);  // a tile row is one 128-byte piece
         if (c.out_cols && lds_dma < 2) c.tpl = std::max(1, c.tpl / 2);
-        if (dma_ok(c, d)) { cfg = c; rec_twl = -1; }
+        bool wide = false;
+        if (lds_dma == 1 && !dbl && inplace && cfg.out_cols && n >= 512) {
+            // In-place FP32 column passes of n >= 512 where a 32-line tile (256-byte pieces of
+            // every row) fits with the LDS twiddle table: the plain kernel, persistent, four
+            // butterflies per thread and pass.  EXHAUSTIVE picked it at 512^3 in three runs,
+            // 416 us against 427 us for the LDS-DMA double buffer (the 32-line tile fits only
+            // up to n = 512; at 256^3 and 384^3 the tuner picked other kernels, which keep the
+            // rule below).
+            fft_config w = cfg;
+            const int rmax = *std::max_element(w.radices.begin(), w.radices.end());
+            w.dma = 0;
+            w.lpb = 32;
+            w.pad = 1;
+            w.tpl = std::max(1, n / rmax / 4);
+            apply_twl(w, d, 1);  // the measured form: persistent with LDS twiddles (>= 2048 tiles)
+            if (w.twl && valid(w)) { cfg = w; rec_twl = -1; wide = true; }
+        }
+        if (!wide && dma_ok(c, d)) { cfg = c; rec_twl = -1; }
     }
     // persistent blocks with the twiddle table in LDS: as the tuning record says (the form it
     // was timed in), else by the rule of the configuration
