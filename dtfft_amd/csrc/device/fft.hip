@@ -486,7 +486,9 @@ fft_r2c_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
             const cx<T> s = a + b, df = a - b;
             const cx<T> w = twmul(df, rtw[k]);           // W^k (Z_k - conj Z_{M-k})
             const cx<T> x = mk(half * (s.x + w.y), half * (s.y - w.x));  // (s - i w) / 2
-            out[obase + (i64)line * d.out_ls + k] = x;
+            // non-temporal (measured on MI355X, 1024 x 512 x 256 FP64 R2C rows: 394 against
+            // 399-403 us; FP32 unchanged within the noise)
+            store_nt(&out[obase + (i64)line * d.out_ls + k], x);
         }
         __syncthreads();
     }

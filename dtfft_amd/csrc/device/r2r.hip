@@ -169,6 +169,13 @@ fft_r2r_kernel(const T* __restrict__ in, T* __restrict__ out, const typename tw_
     // post-process loops stay rolled, which keeps the FP32 form's spills to a few registers
     // (61-72 fully unrolled).
     constexpr int MU = (IN_COLS || OUT_COLS) ? 1 : MIT, XU = (IN_COLS || OUT_COLS) ? 1 : XIT;
+    // Types 3, HALF, M even, columns in: the fold works on pairs (m, M - m).  Measured on
+    // MI355X at 512^3 FP32 (DCT-3): the column-in stages 311-312 / 333-334 against 334 /
+    // 353 us; the row stage and the same pairing of the types-2 untangling were no faster
+    // (the column form of the latter 363 against 324 us, at the 128-VGPR limit).
+    constexpr bool PAIRED = HALF && INV && IN_COLS && M % 2 == 0;
+    constexpr int H = M / 2;
+    constexpr int PIT = (LPB * H + F::NT - 1) / F::NT > 0 ? (LPB * H + F::NT - 1) / F::NT : 1;
     __shared__ cx<T> lds[F::LDS_ELEMS];
     __shared__ T xs[LPB * NX];  // real lines: [line][j], or [j][line] when read across lines
     __shared__ tw_t twl[TWN];
@@ -225,6 +232,42 @@ fft_r2r_kernel(const T* __restrict__ in, T* __restrict__ out, const typename tw_
         const i64 l0 = t * LPB;
         fetch(t + gridDim.x);
         __syncthreads();  // the tile's lines (and, the first time, the twiddles) are in LDS
+        if constexpr (PAIRED) {
+            // Pairs (m, M - m), m < M / 2, share their inputs: with e2 = V_m + conj V_{M-m} and
+            // o2 = (V_m - conj V_{M-m}) conj(W^m), Z_m = e2 + i o2 and, as W^{M-m} = -conj W^m,
+            // Z_{M-m} = conj(e2) + i conj(o2).  Item 0 takes m = 0 (V_0 and V_M real) and the
+            // self-paired M / 2.  The work line holds conj Z (backward transform).
+#pragma unroll 1
+            for (int q = 0; q < PIT; q++) {
+                const int e = (int)threadIdx.x + q * F::NT;
+                if (PIT * F::NT == LPB * H || e < LPB * H) {
+                    const int line = IN_COLS ? e % LPB : e / H, m = IN_COLS ? e / LPB : e % H;
+                    auto xf = [&](i64 j) { return IN_COLS ? xs[j * LPB + line] : xs[line * NX + j]; };
+                    auto V = [&](int j) { return r2r_pre_elem<T, KIND>(xf, (i64)NX, (i64)NX, (i64)j, t4); };
+                    cx<T>* L = lds + line * F::LD;
+                    cx<T> e2, o2;
+                    auto fold = [&](int mm, const cx<T>& a, const cx<T>& b) {
+                        const cx<T> bc = conj(b);
+                        e2 = a + bc;
+                        o2 = twmul(a - bc, conj_tw(rtwl[mm]));
+                    };
+                    if (m == 0) {
+                        cx<T> a = V(0), b = V(M);
+                        a.y = (T)0;
+                        b.y = (T)0;
+                        fold(0, a, b);
+                        L[F::lidx(0)] = mk(e2.x - o2.y, -(e2.y + o2.x));
+                        const cx<T> h = V(H);
+                        fold(H, h, h);
+                        L[F::lidx(H)] = mk(e2.x - o2.y, -(e2.y + o2.x));
+                    } else {
+                        fold(m, V(m), V(M - m));
+                        L[F::lidx(m)] = mk(e2.x - o2.y, -(e2.y + o2.x));
+                        L[F::lidx(M - m)] = mk(e2.x + o2.y, e2.y - o2.x);
+                    }
+                }
+            }
+        } else {
 #pragma unroll MU
         for (int q = 0; q < MIT; q++) {
             const int e = (int)threadIdx.x + q * F::NT;
@@ -251,6 +294,7 @@ fft_r2r_kernel(const T* __restrict__ in, T* __restrict__ out, const typename tw_
                 // backward transform = conj(forward(conj z))
                 lds[line * F::LD + F::lidx(m)] = INV ? conj(z) : z;
             }
+        }
         }
         __syncthreads();
         F::template passes<0, false, false>(nullptr, nullptr, twl, lds, 0, 0, ld, l0, false);
@@ -293,8 +337,13 @@ fft_r2r_kernel(const T* __restrict__ in, T* __restrict__ out, const typename tw_
                         };
                         y = r2r_post_elem<T, KIND>(vf, (i64)NX, (i64)k, t4);
                     }
-                    if constexpr (IN_COLS || OUT_COLS) out[ob + (i64)line * d.out_ls + (i64)k * d.out_es] = y;
-                    else out[li * d.out_ls + k] = y;
+                    // rows out: non-temporal stores (measured on MI355X at 512^3 FP32: the
+                    // column-in / row-out stages 324-326 against 332 us, the row stages
+                    // unchanged within the noise); columns out: plain stores
+                    if constexpr (OUT_COLS) out[ob + (i64)line * d.out_ls + (i64)k * d.out_es] = y;
+                    else if constexpr (IN_COLS)
+                        __builtin_nontemporal_store(y, &out[ob + (i64)line * d.out_ls + (i64)k * d.out_es]);
+                    else __builtin_nontemporal_store(y, &out[li * d.out_ls + k]);
                 }
             }
         }
