@@ -464,12 +464,14 @@ fft_c2c_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
 //   X_k = ((Z_k + conj Z_{M-k}) - i W^k (Z_k - conj Z_{M-k})) / 2,  W = exp(-2 pi i / n),
 // for k = 0..M (Z_M = Z_0).  Strides in `d` are in complex units of the input view
 // (in_ls = n/2) and of the output (out_ls = odist).  rtw holds W^k, k = 0..M.
-template <typename T, class S, int TPL, int LPB, int PAD>
+// OSEG > 0: output k at (k / OSEG) * out_seg_stride + k % OSEG (the chunk-blocked input of a
+// following column-in pass, as fft_c2c_kernel's OSEG).
+template <typename T, class S, int TPL, int LPB, int PAD, int OSEG = 0>
 __global__ void __launch_bounds__(TPL * LPB)
 fft_r2c_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
                const typename tw_of<T>::type* __restrict__ tw,
                const typename tw_of<T>::type* __restrict__ rtw, dtfft::lines_desc d) {
-    typedef dtfft::fft_tile<T, S, TPL, LPB, false, false, PAD> F;
+    typedef dtfft::fft_tile<T, S, TPL, LPB, false, false, PAD, 0, OSEG> F;
     constexpr int M = S::N;
     __shared__ cx<T> lds[F::LDS_ELEMS];
     for (i64 t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
@@ -488,7 +490,7 @@ fft_r2c_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
             const cx<T> x = mk(half * (s.x + w.y), half * (s.y - w.x));  // (s - i w) / 2
             // non-temporal (measured on MI355X, 1024 x 512 x 256 FP64 R2C rows: 394 against
             // 399-403 us; FP32 unchanged within the noise)
-            store_nt(&out[obase + (i64)line * d.out_ls + k], x);
+            store_nt(&out[obase + (i64)line * d.out_ls + F::out_elem(k, d)], x);
         }
         __syncthreads();
     }

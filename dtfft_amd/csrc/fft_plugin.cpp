@@ -74,6 +74,7 @@ std::string fft_config::expr() const {
     if (kind == 0 && (seg > 0 || twl || oseg > 0)) s += ", " + std::to_string(seg);
     if (kind == 0 && (twl || oseg > 0)) s += twl ? ", true" : ", false";
     if (kind == 0 && oseg > 0) s += ", " + std::to_string(oseg);
+    if (kind == 1 && oseg > 0) s += ", " + std::to_string(oseg);
     if (kind == 2 && (in_cols || twl)) s += in_cols ? ", true" : ", false";
     if (kind == 2 && twl) s += ", true";
     s += ">";

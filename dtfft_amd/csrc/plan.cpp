@@ -165,6 +165,50 @@ static int32_t blocked_pair(plan_t* p, const fft_stage& rows, const fft_stage& c
     return DTFFT_HIP_SUCCESS;
 }
 
+// Chunk-blocked pair of the fused 3D R2C forward: the R2C rows write the complex X-pencil in
+// chunks of B lines (element (x, y) of plane z at z * ncp * ny + (x / B) * ny * B + y * B +
+// x % B), padded to ncp = ceil(nc / B) * B lines, so that the X->Y pass runs the LDS-DMA
+// column kernel on whole contiguous tiles although nc = nx / 2 + 1 lines do not divide into
+// them (nor do rows of nc complex values align to 16 bytes).  The padding lines of the last
+// chunk are never written; the X->Y pass transforms them into planes x >= nc of the Y-pencil
+// in aux, which the Y->Z pass does not read.  Built where the padding costs at most 1/16 of
+// the lines (the plan's buffers grow to ncp * ny * nz complex values).
+static int32_t blocked_r2c(plan_t* p, bool* ok) {
+    *ok = false;
+    const int64_t ny = p->ny, nz = p->nz, nc = p->nc, B = 16;
+    const fft_config& rc = p->fx.cfg;
+    if (!global_config().blocked || p->dbl || p->lds_dma == 0 || !p->fx.k || rc.kind != 1 || rc.any || rc.dma)
+        return DTFFT_HIP_SUCCESS;  // the half-length R2C row kernel (fft_r2c_kernel) only
+    const int64_t ncp = (nc + B - 1) / B * B;
+    if ((ncp - nc) * 16 > nc) return DTFFT_HIP_SUCCESS;
+    fft_stage nat;  // the X->Y pass over the padded rows in the natural layout: its configuration
+    TRY(fused(p, ny, ncp, nz, ncp * ny, ncp, ny * nz, ny, &nat));
+    const fft_config& cc = nat.cfg;
+    if (!nat.k || cc.kind != 0 || cc.any || !cc.in_cols || cc.out_cols || cc.seg || !cc.dma || cc.lpb != B)
+        return DTFFT_HIP_SUCCESS;
+    lines_desc bd = nat.d;
+    bd.in_ns = B;
+    bd.in_ts = ny * B;  // planes stay ncp * ny apart
+    fft_config c = cc;
+    // the persistent double-buffer LDS-DMA kernel is the faster form on a blocked input
+    if (c.dma == 2 && p->lds_dma == 1) {
+        fft_config c2 = c;
+        c2.dma = 1;
+        if (dma_ok(c2, bd)) c = c2;
+    }
+    if (!dma_ok(c, bd)) return DTFFT_HIP_SUCCESS;
+    fft_config r = rc;
+    r.oseg = (int)B;
+    const int64_t hl = p->fx.d.in_ls;  // real row length in complex units (nx / 2)
+    lines_desc rd = make_lines(ny, nz, hl, hl * ny, 1, B, ncp * ny, 1, r.lpb);
+    rd.out_seg_stride = ny * B;
+    TRY(p->fx_blk.init(r, rd));
+    TRY(p->u_xy_blk.init(c, bd));
+    p->alloc_elems = std::max(p->alloc_elems, (size_t)(2 * ncp * ny * nz));
+    *ok = true;
+    return DTFFT_HIP_SUCCESS;
+}
+
 // R2R stages along X, Y (and Z), each over the rows of its pencil; the backward stages
 // use the inverse kinds (src/dtfft_utils.F90:447-471).
 static int32_t build_r2r(plan_t* p) {
@@ -275,6 +319,7 @@ static int32_t build(plan_t* p) {
                                &p->fy_zc, p->lds_dma));
             TRY(plan_c2r_cols_stage((int)nx, p->dbl, nz, ny, nz * nc, nz, ny * (nx / 2), nx / 2, &p->u_zx_c2r,
                                     p->lds_dma));
+            TRY(blocked_r2c(p, &p->blk_fwd));
         } else {
             TRY(fused(p, nx, nz, ny, nz * nc, nz, nc * ny, nc, &p->u_zx));  // Z -> X + FFT_X
             TRY(blocked_pair(p, p->fx, p->u_xy, &p->fx_blk, &p->u_xy_blk, &p->blk_fwd));

@@ -10,7 +10,8 @@ kernel's two instantiations (natural and chunked output) may contract a differen
 of a butterfly into an FMA (measured on MI355X: 256-point FP64 rows, outputs past the first
 radix-16 group, 1-ulp differences at 3e-16 of the oracle either way).  Shapes cover the LDS-DMA column kernel
 (FP32, FFT lengths divisible by 8), the plain column kernel (FP64, and lengths such as 20),
-pairs built in one direction only, partial row tiles and 2D plans."""
+pairs built in one direction only, partial row tiles, 2D plans, and the padded pair of the FP32
+R2C forward (R2C rows -> X->Y)."""
 import numpy as np
 import pytest
 
@@ -122,3 +123,49 @@ def test_blocked_2d_matches_natural_and_oracle(dims, prec, capfd):
     xn = run(pn, y, (ny, nx), False)
     assert rel(xb.astype(np.complex128), xn.astype(np.complex128)) < ULPS[prec]
     assert rel(xb.astype(np.complex128), OP.execute_2d(y.astype(np.complex128), False)) < TOL[prec]
+
+
+@pytest.mark.parametrize("dims", [
+    (512, 32, 16),    # nc = 257 complex lines per row, padded to 272 (17 chunks)
+    (480, 64, 8),     # nc = 241, padded to 256
+    (960, 32, 8),     # nc = 481, padded to 496; radix-3 and -5 rows
+])
+def test_blocked_r2c_matches_natural_and_oracle(dims, capfd):
+    """FP32 3D R2C, fused: the R2C rows write the complex X-pencil chunk-blocked and padded to
+    whole 16-line chunks (nc = nx/2 + 1 lines -> ncp), so that the X->Y pass runs the LDS-DMA
+    column kernel; the padding lines are never written and their transforms land in Y-pencil
+    planes that the Y->Z pass does not read.  The plan's buffers grow to ncp * ny * nz complex
+    values.  Forward against the natural layout and the FP64 oracle (src/dtfft_plan.F90's R2C
+    execute sequence, oracle/plan.py), and the round trip."""
+    nx, ny, nz = dims
+    nc = nx // 2 + 1
+    ncp = (nc + 15) // 16 * 16
+    rng = np.random.default_rng(nx + ny + nz)
+    x = rng.standard_normal((nz, ny, nx)).astype(np.float32)
+    plans = {}
+    for blocked in (True, False):
+        D.set_config(enable_fused=True, blocked=blocked)
+        try:
+            plans[blocked] = D.PlanR2C([nx, ny, nz], P.SINGLE, D.Effort.ESTIMATE, D.Executor.HIP)
+        finally:
+            D.set_config()
+    capfd.readouterr()
+    plans[True].report()
+    names = [ln.split()[0] for ln in capfd.readouterr().out.splitlines()[1:] if ln.strip()]
+    assert "fwd_rows" in names, names
+    assert plans[True].alloc_size == max(nx * ny * nz, 2 * ncp * ny * nz)
+    assert plans[False].alloc_size == max(nx * ny * nz, 2 * nc * ny * nz)
+    outs = {}
+    for blocked, p in plans.items():
+        a = D.DeviceArray.from_numpy(x, p.alloc_bytes)
+        b = D.DeviceArray(p.alloc_bytes)
+        c = D.DeviceArray(p.alloc_bytes)
+        p.execute(a, b, D.Execute.FORWARD)
+        z = b.download((ny, nc, nz), np.complex64).astype(np.complex128)
+        p.execute(b, c, D.Execute.BACKWARD)
+        back = c.download((nz, ny, nx), np.float32).astype(np.float64) / x.size
+        outs[blocked] = (z, back)
+    want = OP.execute_generic_r2c(x.astype(np.float64))
+    assert rel(outs[True][0], want) < TOL[P.SINGLE]
+    assert rel(outs[True][0], outs[False][0]) < ULPS[P.SINGLE]
+    assert rel(outs[True][1], x.astype(np.float64)) < 1e-5
