@@ -167,24 +167,26 @@ static int32_t blocked_pair(plan_t* p, const fft_stage& rows, const fft_stage& c
 
 // Chunk-blocked pair of the fused 3D R2C forward: the R2C rows write the complex X-pencil in
 // chunks of B lines (element (x, y) of plane z at z * ncp * ny + (x / B) * ny * B + y * B +
-// x % B), padded to ncp = ceil(nc / B) * B lines, so that the X->Y pass runs the LDS-DMA
-// column kernel on whole contiguous tiles although nc = nx / 2 + 1 lines do not divide into
-// them (nor do rows of nc complex values align to 16 bytes).  The padding lines of the last
-// chunk are never written; the X->Y pass transforms them into planes x >= nc of the Y-pencil
-// in aux, which the Y->Z pass does not read.  Built where the padding costs at most 1/16 of
-// the lines (the plan's buffers grow to ncp * ny * nz complex values).
+// x % B), padded to ncp = ceil(nc / B) * B lines, so that the X->Y pass reads whole contiguous
+// tiles (FP32: by LDS DMA) although nc = nx / 2 + 1 lines do not divide into them (nor do rows
+// of nc complex values align to 16 bytes).  The padding lines of the last chunk are never
+// written; the X->Y pass transforms them into planes x >= nc of the Y-pencil in aux, which the
+// Y->Z pass does not read.  Built where the padding costs at most 1/16 of the lines (the
+// plan's buffers grow to ncp * ny * nz complex values).
 static int32_t blocked_r2c(plan_t* p, bool* ok) {
     *ok = false;
     const int64_t ny = p->ny, nz = p->nz, nc = p->nc, B = 16;
     const fft_config& rc = p->fx.cfg;
-    if (!global_config().blocked || p->dbl || p->lds_dma == 0 || !p->fx.k || rc.kind != 1 || rc.any || rc.dma)
+    if (!global_config().blocked || !p->fx.k || rc.kind != 1 || rc.any || rc.dma)
         return DTFFT_HIP_SUCCESS;  // the half-length R2C row kernel (fft_r2c_kernel) only
     const int64_t ncp = (nc + B - 1) / B * B;
     if ((ncp - nc) * 16 > nc) return DTFFT_HIP_SUCCESS;
     fft_stage nat;  // the X->Y pass over the padded rows in the natural layout: its configuration
     TRY(fused(p, ny, ncp, nz, ncp * ny, ncp, ny * nz, ny, &nat));
+    // the column pass's tile is B lines: FP32 the LDS-DMA kernel (128-byte pieces), FP64 the
+    // plain column kernel (256-byte pieces)
     const fft_config& cc = nat.cfg;
-    if (!nat.k || cc.kind != 0 || cc.any || !cc.in_cols || cc.out_cols || cc.seg || !cc.dma || cc.lpb != B)
+    if (!nat.k || cc.kind != 0 || cc.any || !cc.in_cols || cc.out_cols || cc.seg || cc.lpb != B)
         return DTFFT_HIP_SUCCESS;
     lines_desc bd = nat.d;
     bd.in_ns = B;
@@ -196,7 +198,7 @@ static int32_t blocked_r2c(plan_t* p, bool* ok) {
         c2.dma = 1;
         if (dma_ok(c2, bd)) c = c2;
     }
-    if (!dma_ok(c, bd)) return DTFFT_HIP_SUCCESS;
+    if (c.dma && !dma_ok(c, bd)) return DTFFT_HIP_SUCCESS;
     fft_config r = rc;
     r.oseg = (int)B;
     const int64_t hl = p->fx.d.in_ls;  // real row length in complex units (nx / 2)

@@ -249,7 +249,7 @@ int32_t dtfft_hip_execute_cycle_graph(dtfft_hip_plan_t plan, void* in, void* out
 #define DTFFT_HIP_NO_ALLOC_AUX_BYTES ((size_t)128 << 20)
 int32_t dtfft_hip_execute_no_alloc(dtfft_hip_plan_t plan, void* in, void* out, int32_t execute_type);
 /* starts/counts in reverse order, as dtfft_get_local_sizes; alloc_size in elements.  A fused
-   FP32 3D R2C plan whose forward keeps its complex X-pencil padded to whole 16-line chunks
+   3D R2C plan whose forward keeps its complex X-pencil padded to whole 16-line chunks
    asks for up to 1/16 more than the complex pencil (2 * ceil((nx/2+1)/16)*16 * ny * nz reals). */
 int32_t dtfft_hip_get_local_sizes(dtfft_hip_plan_t plan, int32_t* in_starts, int32_t* in_counts,
                                   int32_t* out_starts, int32_t* out_counts, size_t* alloc_size);

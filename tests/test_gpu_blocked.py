@@ -125,28 +125,30 @@ def test_blocked_2d_matches_natural_and_oracle(dims, prec, capfd):
     assert rel(xb.astype(np.complex128), OP.execute_2d(y.astype(np.complex128), False)) < TOL[prec]
 
 
-@pytest.mark.parametrize("dims", [
-    (512, 32, 16),    # nc = 257 complex lines per row, padded to 272 (17 chunks)
-    (480, 64, 8),     # nc = 241, padded to 256
-    (960, 32, 8),     # nc = 481, padded to 496; radix-3 and -5 rows
+@pytest.mark.parametrize("dims,prec", [
+    ((512, 32, 16), P.SINGLE),    # nc = 257 complex lines per row, padded to 272 (17 chunks)
+    ((480, 64, 8), P.SINGLE),     # nc = 241, padded to 256
+    ((960, 32, 8), P.SINGLE),     # nc = 481, padded to 496; radix-3 and -5 rows
+    ((480, 512, 4), P.DOUBLE),    # FP64: 16-line tiles of the plain column kernel (512 points)
 ])
-def test_blocked_r2c_matches_natural_and_oracle(dims, capfd):
-    """FP32 3D R2C, fused: the R2C rows write the complex X-pencil chunk-blocked and padded to
-    whole 16-line chunks (nc = nx/2 + 1 lines -> ncp), so that the X->Y pass runs the LDS-DMA
-    column kernel; the padding lines are never written and their transforms land in Y-pencil
-    planes that the Y->Z pass does not read.  The plan's buffers grow to ncp * ny * nz complex
-    values.  Forward against the natural layout and the FP64 oracle (src/dtfft_plan.F90's R2C
-    execute sequence, oracle/plan.py), and the round trip."""
+def test_blocked_r2c_matches_natural_and_oracle(dims, prec, capfd):
+    """3D R2C, fused: the R2C rows write the complex X-pencil chunk-blocked and padded to whole
+    16-line chunks (nc = nx/2 + 1 lines -> ncp), so that the X->Y pass reads contiguous tiles
+    (FP32: the LDS-DMA column kernel); the padding lines are never written and their transforms
+    land in Y-pencil planes that the Y->Z pass does not read.  The plan's buffers grow to
+    ncp * ny * nz complex values.  Forward against the natural layout and the FP64 oracle
+    (src/dtfft_plan.F90's R2C execute sequence, oracle/plan.py), and the round trip."""
     nx, ny, nz = dims
     nc = nx // 2 + 1
     ncp = (nc + 15) // 16 * 16
+    rdt = np.float64 if prec == P.DOUBLE else np.float32
     rng = np.random.default_rng(nx + ny + nz)
-    x = rng.standard_normal((nz, ny, nx)).astype(np.float32)
+    x = rng.standard_normal((nz, ny, nx)).astype(rdt)
     plans = {}
     for blocked in (True, False):
         D.set_config(enable_fused=True, blocked=blocked)
         try:
-            plans[blocked] = D.PlanR2C([nx, ny, nz], P.SINGLE, D.Effort.ESTIMATE, D.Executor.HIP)
+            plans[blocked] = D.PlanR2C([nx, ny, nz], prec, D.Effort.ESTIMATE, D.Executor.HIP)
         finally:
             D.set_config()
     capfd.readouterr()
@@ -161,11 +163,11 @@ def test_blocked_r2c_matches_natural_and_oracle(dims, capfd):
         b = D.DeviceArray(p.alloc_bytes)
         c = D.DeviceArray(p.alloc_bytes)
         p.execute(a, b, D.Execute.FORWARD)
-        z = b.download((ny, nc, nz), np.complex64).astype(np.complex128)
+        z = b.download((ny, nc, nz), cdt(prec)).astype(np.complex128)
         p.execute(b, c, D.Execute.BACKWARD)
-        back = c.download((nz, ny, nx), np.float32).astype(np.float64) / x.size
+        back = c.download((nz, ny, nx), rdt).astype(np.float64) / x.size
         outs[blocked] = (z, back)
     want = OP.execute_generic_r2c(x.astype(np.float64))
-    assert rel(outs[True][0], want) < TOL[P.SINGLE]
-    assert rel(outs[True][0], outs[False][0]) < ULPS[P.SINGLE]
-    assert rel(outs[True][1], x.astype(np.float64)) < 1e-5
+    assert rel(outs[True][0], want) < TOL[prec]
+    assert rel(outs[True][0], outs[False][0]) < ULPS[prec]
+    assert rel(outs[True][1], x.astype(np.float64)) < TOL[prec] * 10
