@@ -105,7 +105,7 @@ struct fft_config {
     int dma = 0;  // column-in passes through LDS filled by direct loads: 1 double buffer
                   // (fft_cols_dma_kernel), 2 single buffer, two blocks per CU (fft_cols_dma1_kernel)
     int twl = 0;  // C2C (kind 0, dma 0): persistent blocks with the twiddle table in LDS
-    int oseg = 0;  // C2C rows (kind 0, dma 0): FFT axis written in chunks of `oseg` elements
+    int oseg = 0;  // C2C and R2C rows (kind 0 / 1, dma 0): FFT axis written in chunks of `oseg` elements
     // fft_any_kernel (device/anylen.hip): any == 1 runs a line of logical length nlog through
     // the n-point schedule in LDS, directly (chirp false, n == nlog) or by Bluestein's chirp
     // z-transform (chirp true, n >= 2 nlog - 1); kind 0 / 1 / 2 = C2C / R2C / C2R of nlog
