@@ -166,9 +166,11 @@ fft_r2r_kernel(const T* __restrict__ in, T* __restrict__ out, const typename tw_
     constexpr int MIT = (LPB * M + F::NT - 1) / F::NT;   // work elements per thread
     constexpr int TWN = S::TW_SIZE > 0 ? S::TW_SIZE : 1;
     // The column forms run 1024-thread blocks at the 128-VGPR limit: their pre- and
-    // post-process loops stay rolled, which keeps the FP32 form's spills to a few registers
-    // (61-72 fully unrolled).
-    constexpr int MU = (IN_COLS || OUT_COLS) ? 1 : MIT, XU = (IN_COLS || OUT_COLS) ? 1 : XIT;
+    // post-process loops are unrolled twice, which keeps the FP32 form's spills to a few
+    // registers (61-72 fully unrolled).  Measured on MI355X at 512^3 FP32: the column stages
+    // 313 / 309 / 328 us unrolled twice, against 322 / 312 / 330 us rolled and 313 / 307 /
+    // 344 us unrolled four times.
+    constexpr int MU = (IN_COLS || OUT_COLS) ? 2 : MIT, XU = (IN_COLS || OUT_COLS) ? 2 : XIT;
     // Types 3, HALF, M even, columns in: the fold works on pairs (m, M - m).  Measured on
     // MI355X at 512^3 FP32 (DCT-3): the column-in stages 311-312 / 333-334 against 334 /
     // 353 us; the row stage and the same pairing of the types-2 untangling were no faster
