@@ -480,17 +480,30 @@ fft_r2c_kernel(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
         F::template passes<0, true, false>(in, out, tw, lds, ibase, obase, d, l0, false);
         __syncthreads();
         const T half = (T)0.5;
-        for (int e = threadIdx.x; e < LPB * (M + 1); e += F::NT) {
-            const int line = e / (M + 1), k = e % (M + 1);
-            if (l0 + line >= d.nl) continue;
+        auto untangle = [&](int line, int k) {
             const cx<T> a = lds[line * F::LD + F::lidx(k == M ? 0 : k)];
             const cx<T> b = conj(lds[line * F::LD + F::lidx(k == 0 ? 0 : M - k)]);
             const cx<T> s = a + b, df = a - b;
-            const cx<T> w = twmul(df, rtw[k]);           // W^k (Z_k - conj Z_{M-k})
+            const cx<T> w = twmul(df, rtw[k]);                            // W^k (Z_k - conj Z_{M-k})
             const cx<T> x = mk(half * (s.x + w.y), half * (s.y - w.x));  // (s - i w) / 2
             // non-temporal (measured on MI355X, 1024 x 512 x 256 FP64 R2C rows: 394 against
             // 399-403 us; FP32 unchanged within the noise)
             store_nt(&out[obase + (i64)line * d.out_ls + F::out_elem(k, d)], x);
+        };
+        // LPB * M items, fully unrolled; the thread of k = 0 also writes X_M, so the loop has
+        // no mostly idle last round (measured on MI355X, 1024 x 512 x 256: FP32 rows 239
+        // against 254 us, FP64 380 against 397 us for a loop over LPB * (M + 1) items)
+        constexpr int UIT = (LPB * M + F::NT - 1) / F::NT;
+#pragma unroll
+        for (int q = 0; q < UIT; q++) {
+            const int e = (int)threadIdx.x + q * F::NT;
+            if (UIT * F::NT == LPB * M || e < LPB * M) {
+                const int line = e / M, k = e % M;
+                if (l0 + line < d.nl) {
+                    untangle(line, k);
+                    if (k == 0) untangle(line, M);
+                }
+            }
         }
         __syncthreads();
     }
