@@ -135,3 +135,35 @@ def test_force_effort_estimate_ignores_tuning_records(D, tmp_path, capfd, monkey
         assert f"{key}: tile 64 rows 16 pad 1 vec 2 (default)" in err, err
     finally:
         L.dtfft_hip_set_cache_dir(default_dir.encode())
+
+
+def test_compile_only_r2c_padded_pair_layout(D, capfd):
+    """The fused 3D R2C forward builds the padded chunk-blocked pair where the padding is at
+    most 1/16 of the lines (nc = 257 -> 272 here) and reports the larger buffers; where the
+    padding would cost more (nc = 9 -> 16) or fusion is off, the natural layout and size stay."""
+    L = D.lib()
+    if L.dtfft_hip_get_device_arch is None:  # pragma: no cover
+        pytest.skip()
+    if D.device_arch().startswith("gfx950"):
+        pytest.skip("a GPU is present; compile-only behaviour is exercised on CPU hosts")
+
+    def plan(dims, fused):
+        D.set_config(enable_fused=fused)
+        try:
+            p = D.PlanR2C(dims, D.Precision.SINGLE, D.Effort.ESTIMATE, D.Executor.HIP)
+        finally:
+            D.set_config()
+        capfd.readouterr()
+        p.report()
+        return p, [ln.split()[0] for ln in capfd.readouterr().out.splitlines()[1:] if ln.strip()]
+
+    nx, ny, nz = 512, 32, 16
+    p, names = plan([nx, ny, nz], True)
+    assert "fwd_rows" in names and "fwd_cols" in names, names
+    assert p.alloc_size == 2 * 272 * ny * nz
+    q, names = plan([nx, ny, nz], False)
+    assert "fwd_rows" not in names
+    assert q.alloc_size == max(nx * ny * nz, 2 * 257 * ny * nz)
+    r, names = plan([16, 32, 16], True)
+    assert "fwd_rows" not in names
+    assert r.alloc_size == max(16 * 32 * 16, 2 * 9 * 32 * 16)
