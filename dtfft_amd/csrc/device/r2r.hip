@@ -30,6 +30,12 @@ struct r2r_desc {
     i64 out_ls, out_bs, out_es;
     i64 n, m;                  // real length, complex work length
     i64 wdist;                 // complex work line stride (>= m)
+    // Chunk-blocked forms of the fused kernel (0: natural layout).  in_ts: columns in, tile t
+    // of a batch starts at t * in_ts (its LPB lines one contiguous block, element j of line l
+    // at j * in_es + l).  out_seg: rows out with OSEG > 0, element k of a line at
+    // (k / OSEG) * out_seg + k % OSEG from the line's start.
+    i64 in_ts;
+    i64 out_seg;
 };
 
 template <typename T>
@@ -151,7 +157,7 @@ struct r2r_len {
 // M-point transform is v_2j + i v_2j+1 (the C2R folding of fft_c2r_kernel).  rtwtab holds
 // W^k, k <= M (gen_rtwiddles); it is unused otherwise.
 template <typename T, class S, int TPL, int LPB, int PAD, int KIND, bool FULL, bool HALF = false,
-          bool IN_COLS = false, bool OUT_COLS = false>
+          bool IN_COLS = false, bool OUT_COLS = false, int OSEG = 0>
 __global__ void __launch_bounds__(TPL * LPB)
 fft_r2r_kernel(const T* __restrict__ in, T* __restrict__ out, const typename tw_of<T>::type* __restrict__ tw,
                const cx<T>* __restrict__ t4tab, const typename tw_of<T>::type* __restrict__ rtwtab, r2r_desc d) {
@@ -162,6 +168,7 @@ fft_r2r_kernel(const T* __restrict__ in, T* __restrict__ out, const typename tw_
     constexpr bool INV = (KIND == 4 || KIND == 8);  // types 3 run the backward transform
     static_assert(!HALF || KIND == 4 || KIND == 5 || KIND == 8 || KIND == 9, "half length: types 2 and 3");
     static_assert(!(IN_COLS || OUT_COLS) || FULL, "column layouts run whole tiles");
+    static_assert(OSEG == 0 || (!IN_COLS && !OUT_COLS), "chunked output: the row form");
     constexpr int XIT = (LPB * NX + F::NT - 1) / F::NT;  // real elements per thread
     constexpr int MIT = (LPB * M + F::NT - 1) / F::NT;   // work elements per thread
     constexpr int TWN = S::TW_SIZE > 0 ? S::TW_SIZE : 1;
@@ -195,7 +202,10 @@ fft_r2r_kernel(const T* __restrict__ in, T* __restrict__ out, const typename tw_
     // j, lines in_ls apart (the host folds evenly spaced batches into one range).  Columns
     // (IN_COLS / OUT_COLS): LPB adjacent lines (ls 1), element j at j * es; nl is a multiple
     // of LPB, so a tile never straddles two batches.
-    auto in_base = [&](i64 l0) { const i64 b = l0 / d.nl; return b * d.in_bs + (l0 - b * d.nl) * d.in_ls; };
+    auto in_base = [&](i64 l0) {
+        const i64 b = l0 / d.nl;
+        return b * d.in_bs + (d.in_ts ? (l0 - b * d.nl) / LPB * d.in_ts : (l0 - b * d.nl) * d.in_ls);
+    };
     auto out_base = [&](i64 l0) { const i64 b = l0 / d.nl; return b * d.out_bs + (l0 - b * d.nl) * d.out_ls; };
     // element q of this thread in a tile: rows (line, j) = ((tid + q NT) / NX, % NX); columns
     // (line, j) = ((tid + q NT) % LPB, / LPB), so a wave reads whole LPB-line pieces of rows
@@ -301,7 +311,9 @@ fft_r2r_kernel(const T* __restrict__ in, T* __restrict__ out, const typename tw_
         __syncthreads();
         F::template passes<0, false, false>(nullptr, nullptr, twl, lds, 0, 0, ld, l0, false);
         __syncthreads();
-        const i64 ob = (IN_COLS || OUT_COLS) ? out_base(l0) : 0;
+        // chunked rows (OSEG): the host keeps nl a multiple of LPB, so a tile lies in one batch
+        // and its lines' bases follow from the tile's (no division per store)
+        const i64 ob = (IN_COLS || OUT_COLS || OSEG > 0) ? out_base(l0) : 0;
 #pragma unroll XU
         for (int q = 0; q < XIT; q++) {
             const int e = (int)threadIdx.x + q * F::NT;
@@ -345,6 +357,8 @@ fft_r2r_kernel(const T* __restrict__ in, T* __restrict__ out, const typename tw_
                     if constexpr (OUT_COLS) out[ob + (i64)line * d.out_ls + (i64)k * d.out_es] = y;
                     else if constexpr (IN_COLS)
                         __builtin_nontemporal_store(y, &out[ob + (i64)line * d.out_ls + (i64)k * d.out_es]);
+                    else if constexpr (OSEG > 0)
+                        __builtin_nontemporal_store(y, &out[ob + (i64)line * d.out_ls + (i64)(k / OSEG) * d.out_seg + k % OSEG]);
                     else __builtin_nontemporal_store(y, &out[li * d.out_ls + k]);
                 }
             }

@@ -174,6 +174,8 @@ struct r2r_desc_h {
     int64_t out_ls, out_bs, out_es;
     int64_t n, m;
     int64_t wdist;
+    int64_t in_ts;    // chunk-blocked column input: tile stride (0: natural)
+    int64_t out_seg;  // chunk-blocked row output: chunk stride (kernel built with OSEG)
 };
 // One R2R transform (DCT/DST 1-4, FFTW kinds 3..10) over nl x nb real lines of length n:
 // pre-process -> complex FFT of length m -> post-process.  Lines of unit element stride
@@ -204,6 +206,15 @@ struct r2r_stage {
     // multiple of the 128-byte tile row.
     int32_t init_cols(int kind, int64_t n, bool dbl, int64_t nl, int64_t nb, int64_t in_bs, int64_t in_es,
                       int64_t out_ls, int64_t out_bs, int64_t out_es, bool out_cols, bool* ok);
+    // Chunk-blocked pair (fused kernels, half-length form): `rows` writing its lines (nl
+    // lines of nb batches, rows of n) in chunks of B elements -- element k of line (l, b) at
+    // b * n * nl + l * B + (k / B) * nl * B + k % B -- so that the column stage `cols` (B lines
+    // per tile, FFT length nl, lines n per batch) reads each tile as one contiguous block.
+    // *ok is false (nothing set up) where the stages have no such form.
+    static int32_t blocked_pair(const r2r_stage& rows, const r2r_stage& cols, r2r_stage* rows_blk,
+                                r2r_stage* cols_blk, bool* ok);
+    std::string expr;  // the fused kernel's name expression
+    int lpb = 0;       // lines per tile of the fused kernel
     size_t work_bytes() const;
     int32_t run(const void* in, void* out, void* work, hipStream_t s) const;
 };

@@ -73,6 +73,10 @@ struct plan_t {
     // fused R2R sequence: the transposes folded into the stages reading across lines
     r2r_stage ryc, rzc, rxbc, rybc;
     bool r2r_fused = false;
+    // chunk-blocked pairs of the fused R2R sequence: forward rx -> ryc, backward rzb -> rxbc
+    // (3D) / ryb -> rxbc (2D)
+    r2r_stage rx_blk, ryc_blk, rb_blk, rbc_blk;
+    bool r2r_blk_fwd = false, r2r_blk_bwd = false;
     void* work = nullptr;                            // complex work lines of the R2R stages
     // forward + backward captured as one hipGraph for one set of pointers
     hipStream_t gstream = nullptr;
@@ -245,10 +249,18 @@ static int32_t build_r2r(plan_t* p) {
             TRY(p->rxbc.init_cols(i0, nx, p->dbl, nz, ny, nz * nx, nz, nx * ny, nx, 1, false, &c));
             TRY(p->rybc.init_cols(i1, ny, p->dbl, nx, nz, nx * ny, nx, 1, nx * ny, nx, true, &e));
             p->r2r_fused = a && b && c && e;
+            if (p->r2r_fused) {
+                TRY(r2r_stage::blocked_pair(p->rx, p->ryc, &p->rx_blk, &p->ryc_blk, &p->r2r_blk_fwd));
+                TRY(r2r_stage::blocked_pair(p->rzb, p->rxbc, &p->rb_blk, &p->rbc_blk, &p->r2r_blk_bwd));
+            }
         } else {
             TRY(p->ryc.init_cols(k1, ny, p->dbl, nx, 1, 0, nx, ny, 0, 1, false, &a));
             TRY(p->rxbc.init_cols(i0, nx, p->dbl, ny, 1, 0, ny, nx, 0, 1, false, &c));
             p->r2r_fused = a && c;
+            if (p->r2r_fused) {
+                TRY(r2r_stage::blocked_pair(p->rx, p->ryc, &p->rx_blk, &p->ryc_blk, &p->r2r_blk_fwd));
+                TRY(r2r_stage::blocked_pair(p->ryb, p->rxbc, &p->rb_blk, &p->rbc_blk, &p->r2r_blk_bwd));
+            }
         }
     }
     return DTFFT_HIP_SUCCESS;
@@ -465,21 +477,25 @@ static int32_t exec_r2r(plan_t* p, void* in, void* out, bool fwd, void* aux, ste
     void* w = p->work;
     TRY(ensure_aux(p, aux));
     if (p->r2r_fused) {
+        // the chunk-blocked pairs write their intermediate into aux, or into out (3D forward,
+        // when out is not the input)
+        const bool bf = p->r2r_blk_fwd, bb = p->r2r_blk_bwd;
         if (p->ndims == 2) {
             if (fwd) {
-                STEP(r, p->rx.run(in, aux, w, s));
-                STEP(r, p->ryc.run(aux, out, w, s));
+                STEP(r, (bf ? p->rx_blk : p->rx).run(in, aux, w, s));
+                STEP(r, (bf ? p->ryc_blk : p->ryc).run(aux, out, w, s));
             } else {
-                STEP(r, p->ryb.run(in, aux, w, s));
-                STEP(r, p->rxbc.run(aux, out, w, s));
+                STEP(r, (bb ? p->rb_blk : p->ryb).run(in, aux, w, s));
+                STEP(r, (bb ? p->rbc_blk : p->rxbc).run(aux, out, w, s));
             }
         } else if (fwd) {
-            STEP(r, p->rx.run(in, out, w, s));
-            STEP(r, p->ryc.run(out, aux, w, s));
+            const bool b3 = bf && in != out;
+            STEP(r, (b3 ? p->rx_blk : p->rx).run(in, out, w, s));
+            STEP(r, (b3 ? p->ryc_blk : p->ryc).run(out, aux, w, s));
             STEP(r, p->rzc.run(aux, out, w, s));
         } else {
-            STEP(r, p->rzb.run(in, aux, w, s));
-            STEP(r, p->rxbc.run(aux, out, w, s));
+            STEP(r, (bb ? p->rb_blk : p->rzb).run(in, aux, w, s));
+            STEP(r, (bb ? p->rbc_blk : p->rxbc).run(aux, out, w, s));
             STEP(r, p->rybc.run(out, out, w, s));
         }
         return DTFFT_HIP_SUCCESS;
@@ -800,6 +816,8 @@ int32_t dtfft_hip_report(dtfft_hip_plan_t plan) {
     if (p->r2r_fused) {
         show_r2r("xy+r2r", p->ryc); show_r2r("yz+r2r", p->rzc); show_r2r("zx+r2r", p->rxbc);
         show_r2r("r2r_ycol", p->rybc);
+        if (p->r2r_blk_fwd) { show_r2r("r2r_frow", p->rx_blk); show_r2r("r2r_fcol", p->ryc_blk); }
+        if (p->r2r_blk_bwd) { show_r2r("r2r_brow", p->rb_blk); show_r2r("r2r_bcol", p->rbc_blk); }
     }
     auto show = [](const char* name, const fft_stage& st) {
         if (!st.k) return;

@@ -112,6 +112,8 @@ int32_t r2r_stage::init(int kind_, int64_t n_, bool dbl_, int64_t nl, int64_t nb
                  std::to_string(kind) + ((nl * nb) % c.lpb == 0 ? ", true" : ", false") +
                  (half_ok ? ", true>" : ">");
             TRY(rtc_get("r2r", e, &fk));
+            expr = e;
+            lpb = c.lpb;
             TRY(get_twiddles((int)mf, c.radices, dbl, &ftw));
             TRY(get_r2r_table(n, dbl, &t4));
             if (half_ok) TRY(get_rtwiddles((int)n, dbl, &rtw));
@@ -133,6 +135,48 @@ int32_t r2r_stage::init(int kind_, int64_t n_, bool dbl_, int64_t nl, int64_t nb
     TRY(rtc_get("r2r", "r2r_pre<" + t + ", " + std::to_string(kind) + ">", &pre));
     TRY(rtc_get("r2r", "r2r_post<" + t + ", " + std::to_string(kind) + ">", &post));
     return plan_fft_stage((int)m, dbl, nl * nb, 1, m, 0, 1, m, 0, 1, true, effort, stream, &fft);
+}
+
+int32_t r2r_stage::blocked_pair(const r2r_stage& rows, const r2r_stage& cols, r2r_stage* rows_blk,
+                                r2r_stage* cols_blk, bool* ok) {
+    *ok = false;
+    if (!global_config().blocked) return DTFFT_HIP_SUCCESS;
+    // rows: the fused half-length row form over evenly spaced contiguous rows (batches folded)
+    if (!rows.fused || !rows.half || rows.d.nb != 1 || rows.d.in_es != 1 || rows.d.out_es != 1 ||
+        rows.d.in_ls != rows.n || rows.d.out_ls != rows.n || rows.expr.empty() || rows.expr.back() != '>')
+        return DTFFT_HIP_SUCCESS;
+    // cols: the fused column form reading the rows' output across lines: n lines side by side,
+    // its FFT axis (nlc rows) n elements apart, batches of whole planes
+    const int64_t n = rows.n, nlc = cols.n, B = cols.lpb;
+    const r2r_desc_h& cd = cols.d;
+    if (!cols.fused || !cols.half || cols.dbl != rows.dbl || B <= 0 || n % B != 0 || cd.nl != n || cd.in_ls != 1 ||
+        cd.in_es != n || cd.in_ts != 0 || (cd.nb > 1 && cd.in_bs != n * nlc) || rows.d.nl != nlc * cd.nb)
+        return DTFFT_HIP_SUCCESS;
+    if (rows.lpb <= 0 || nlc % rows.lpb != 0) return DTFFT_HIP_SUCCESS;  // row tiles within one batch
+    r2r_stage r = rows;
+    r.expr = rows.expr.substr(0, rows.expr.size() - 1) + ", false, false, " + std::to_string(B) + ">";
+    TRY(rtc_get("r2r", r.expr, &r.fk));
+    r.d.nl = nlc;
+    r.d.nb = cd.nb;
+    r.d.in_bs = n * nlc;  // rows stay evenly spaced: line (l, b) at (l + nlc b) n
+    r.d.out_ls = B;
+    r.d.out_bs = n * nlc;
+    r.d.out_seg = nlc * B;
+    int per_cu = 1;
+    if (!compile_only() && r.fk && r.fk->fn &&
+        hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, r.fk->fn, (int)r.fblock, 0) != hipSuccess)
+        per_cu = 1;
+    per_cu = std::max(1, per_cu);
+    const int64_t tiles = (nlc * cd.nb + r.lpb - 1) / r.lpb;
+    r.fgrid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)per_cu * std::max(1, device().num_cus)));
+    r2r_stage c = cols;
+    c.d.in_es = B;
+    c.d.in_ts = nlc * B;
+    c.d.in_bs = n * nlc;
+    *rows_blk = r;
+    *cols_blk = c;
+    *ok = true;
+    return DTFFT_HIP_SUCCESS;
 }
 
 size_t r2r_stage::work_bytes() const { return fused ? 0 : (size_t)(d.nl * d.nb * d.wdist) * (dbl ? 16 : 8); }
@@ -169,6 +213,8 @@ int32_t r2r_stage::init_cols(int kind_, int64_t n_, bool dbl_, int64_t nl, int64
     e += ">, " + std::to_string(c.tpl) + ", " + std::to_string(c.lpb) + ", " + std::to_string(c.pad) + ", " +
          std::to_string(kind) + ", true, true, true" + (out_cols ? ", true>" : ", false>");
     TRY(rtc_get("r2r", e, &fk));
+    expr = e;
+    this->lpb = c.lpb;
     TRY(get_twiddles((int)mf, c.radices, dbl, &ftw));
     TRY(get_r2r_table(n, dbl, &t4));
     TRY(get_rtwiddles((int)n, dbl, &rtw));

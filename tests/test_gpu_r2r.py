@@ -247,3 +247,58 @@ def test_plan_r2r_fused_falls_back_to_permutes(dims, kinds, capfd):
     p.execute(a, b, D.Execute.FORWARD)
     z = b.download((ny, nx, nz), x.dtype).astype(np.float64)
     assert rel(z, R.execute_generic_r2r(x.astype(np.float64), kinds, True)) < TOL[P.SINGLE]
+
+
+def _plan_r2r_blocked(dims, kinds, prec, blocked, capfd):
+    D.set_config(enable_fused=True, blocked=blocked)
+    try:
+        p = D.PlanR2R(dims, kinds, prec, D.Effort.ESTIMATE, D.Executor.HIP)
+    finally:
+        D.set_config()
+    capfd.readouterr()
+    p.report()
+    return p, [ln.split()[0] for ln in capfd.readouterr().out.splitlines()[1:] if ln.strip()]
+
+
+@pytest.mark.parametrize("prec", [P.SINGLE, P.DOUBLE])
+@pytest.mark.parametrize("dims,kinds", [((64, 64, 64), (5, 5, 5)), ((128, 64, 64), (4, 9, 8)),
+                                        ((256, 256), (5, 4)), ((64, 128), (9, 8))])
+def test_plan_r2r_blocked_pairs(dims, kinds, prec, capfd):
+    """The fused R2R sequence with chunk-blocked pairs (the row stage writes its lines in chunks
+    of the column stage's tile width, so that the column stage reads each tile as one contiguous
+    block: forward R2R_X rows -> X->Y + R2R_Y, backward R2R_Z (3D) / R2R_Y (2D) rows -> the
+    column stage into X) against the same sequence in the natural layout and the oracle's
+    reference sequence, both directions; forward in place falls back to the natural pair.
+    Shapes where the row stage's tiles stay within one plane (its lines per block divide the
+    plane's lines), as the pair requires."""
+    p3 = len(dims) == 3
+    shape_in = tuple(reversed(dims))
+    shape_out = (dims[1], dims[0], dims[2]) if p3 else (dims[0], dims[1])
+    rng = np.random.default_rng(sum(dims) + 11)
+    pb, names = _plan_r2r_blocked(dims, kinds, prec, True, capfd)
+    pn, names_n = _plan_r2r_blocked(dims, kinds, prec, False, capfd)
+    assert "r2r_frow" in names and "r2r_bcol" in names, names
+    assert "r2r_frow" not in names_n and "r2r_brow" not in names_n
+    x = rng.standard_normal(shape_in).astype(rdt(prec))
+    outs = {}
+    for name, p in (("blocked", pb), ("natural", pn)):
+        a = D.DeviceArray.from_numpy(x)
+        b = D.DeviceArray(x.nbytes)
+        p.execute(a, b, D.Execute.FORWARD)
+        outs[name] = b.download(shape_out, x.dtype).astype(np.float64)
+    ref = R.execute_generic_r2r if p3 else R.execute_2d_r2r
+    assert rel(outs["blocked"], ref(x.astype(np.float64), kinds, True)) < TOL[prec]
+    assert rel(outs["blocked"], outs["natural"]) < TOL[prec] / 4
+    y = rng.standard_normal(shape_out).astype(rdt(prec))
+    for name, p in (("blocked", pb), ("natural", pn)):
+        a = D.DeviceArray.from_numpy(y)
+        b = D.DeviceArray(y.nbytes)
+        p.execute(a, b, D.Execute.BACKWARD)
+        outs[name] = b.download(shape_in, y.dtype).astype(np.float64)
+    assert rel(outs["blocked"], ref(y.astype(np.float64), kinds, False)) < TOL[prec]
+    assert rel(outs["blocked"], outs["natural"]) < TOL[prec] / 4
+    if p3:
+        a = D.DeviceArray.from_numpy(x)
+        pb.execute(a, a, D.Execute.FORWARD)
+        z = a.download(shape_out, x.dtype).astype(np.float64)
+        assert rel(z, ref(x.astype(np.float64), kinds, True)) < TOL[prec]
