@@ -167,3 +167,31 @@ def test_compile_only_r2c_padded_pair_layout(D, capfd):
     r, names = plan([16, 32, 16], True)
     assert "fwd_rows" not in names
     assert r.alloc_size == max(16 * 32 * 16, 2 * 9 * 32 * 16)
+
+
+def test_compile_only_r2r_blocked_pairs(D, capfd):
+    """The fused R2R sequence builds its chunk-blocked row -> column pairs where the row stage's
+    tiles stay within one plane (128 x 64 x 64 FP32: 32-line forward row tiles over 64 lines
+    per plane, 64-line backward ones over 128) and keeps the natural pair where they would
+    straddle planes (64 x 32 x 96: 64-line row tiles over 32 lines per plane) or where
+    `blocked` is off."""
+    L = D.lib()
+    if L.dtfft_hip_get_device_arch is None:  # pragma: no cover
+        pytest.skip()
+    if D.device_arch().startswith("gfx950"):
+        pytest.skip("a GPU is present; compile-only behaviour is exercised on CPU hosts")
+
+    def names(dims, blocked):
+        D.set_config(enable_fused=True, blocked=blocked)
+        try:
+            p = D.PlanR2R(dims, [5] * len(dims), D.Precision.SINGLE, D.Effort.ESTIMATE, D.Executor.HIP)
+        finally:
+            D.set_config()
+        capfd.readouterr()
+        p.report()
+        return [ln.split()[0] for ln in capfd.readouterr().out.splitlines()[1:] if ln.strip()]
+
+    n = names([128, 64, 64], True)
+    assert {"r2r_frow", "r2r_fcol", "r2r_brow", "r2r_bcol"} <= set(n), n
+    assert not {"r2r_frow", "r2r_brow"} & set(names([128, 64, 64], False))
+    assert not {"r2r_frow", "r2r_brow"} & set(names([64, 32, 96], True))
