@@ -111,7 +111,7 @@ def log(msg):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the benchmark runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the multi-GPU exchange is out of scope here)", flush=True)
         sys.exit(2)
     import dtfft_amd as D
